@@ -1,0 +1,113 @@
+/*
+ * sincdvr.h -- C-ABI of the MI355X (gfx950) sinc-DVR orbital-operator library.
+ *
+ * This is the drop-in boundary for the SINCDVR project routines of
+ * LBNL-AMO-MCTDHF.  The reference's Fortran driver calls, unchanged, a thin
+ * iso_c_binding shim (v1-1_amd/fortran/sincdvr_shim.f90) that forwards to the
+ * entry points below.  Arrays are the reference's DATATYPE arrays: complex*16,
+ * column-major (first index fastest), interleaved (re, im) doubles, passed as
+ * plain pointers.  Sizes follow the reference's explicit-shape dummies.
+ *
+ * Every entry point returns SDV_OK (0) or a negative SDV_ERR_* code; the
+ * message is available from sdv_last_error().  The reference has no return
+ * codes: its fatal conditions print and call mpistop (CFLST,
+ * S/Definitions.ALL:45); the Fortran shim maps a nonzero code to the same stop.
+ *
+ * Host-pointer entry points are synchronous (all device work done on return).
+ * The *_dev variants take device pointers and a hipStream_t (as void*) and
+ * return after enqueueing; the caller synchronises the stream.
+ */
+#ifndef SINCDVR_H
+#define SINCDVR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDV_OK 0
+#define SDV_ERR_ARG (-1)         /* bad argument (size, range, null)          */
+#define SDV_ERR_HIP (-2)         /* HIP runtime error                         */
+#define SDV_ERR_UNSUPPORTED (-3) /* grid size / option not built              */
+#define SDV_ERR_NOMEM (-4)       /* device allocation failed                  */
+#define SDV_ERR_NODEVICE (-5)    /* no gfx950 device visible                  */
+
+typedef struct sdv_ctx sdv_ctx; /* opaque: one grid, resident kernel + KE tables */
+
+/* Project parameters: the subset of `myparams` (SD/sinc_params.f90:6-102)
+ * that the hot path reads, plus the build flavour. */
+typedef struct sdv_config {
+  int numpoints;          /* n: points per axis, cube only (numpoints(1:3), :40)        */
+  double spacing;         /* grid spacing in bohr (:41)                                 */
+  int toothnbig;          /* 240 by default (:33)                                       */
+  int toothnsmall;        /* 40 by default (:32)                                        */
+  int numspf;             /* orbitals per call (myparams numspf)                        */
+  int cnormflag;          /* 0 hermnorm (CONJUGATE=conjg), 1 ECS c-norm (CONJUGATE=)    */
+  int scalingflag;        /* 1 = smooth exterior complex scaling (:54)                  */
+  double scalingtheta;    /* :57                                                        */
+  double scalingstretch;  /* :58                                                        */
+  double scalingdistance; /* :55                                                        */
+  double smoothness;      /* :56                                                        */
+  int notwoflag;          /* call_twoe_matel00 returns zeros (SD/coreproject.f90:1021)  */
+  int debugflag;          /* 4040/4242/4343: op_tinv returns zeros (:260-263)           */
+  int precision;          /* 0: FP64 plain drop-in; 1: FP32 butterflies (mixed form)    */
+} sdv_config;
+
+/* Fill `cfg` with the reference defaults (SD/sinc_params.f90). */
+void sdv_config_default(sdv_config* cfg);
+
+/* Replaces the project-side set-up of myprojectalloc / init_project /
+ * get_twoe_new (SD/sincproject.f90:46-113,311-338; SD/init_sinc_new.f90:395-529):
+ * builds on the device the Coulomb kernel threed_two, its circulant spectrum,
+ * the sinc KE stencil and the ECS Jacobian tables. */
+int sdv_init(const sdv_config* cfg, sdv_ctx** ctx);
+int sdv_free(sdv_ctx* ctx);
+const char* sdv_last_error(void);
+int sdv_set_precision(sdv_ctx* ctx, int precision);
+
+/* op_tinv (module tinvsubmod, SD/coreproject.f90:250-258):
+ *   twoeden03(totpoints, allsize) in, twoereduced(totpoints, allsize) out.
+ *   allsize must be a multiple of circsize (:387-391). */
+int sdv_op_tinv(sdv_ctx* ctx, const double* twoeden03, double* twoereduced, int allsize, int circsize);
+int sdv_op_tinv_dev(sdv_ctx* ctx, const void* twoeden03, void* twoereduced, int allsize, int circsize,
+                    void* stream);
+
+/* call_twoe_matel00 (SD/coreproject.f90:945-955):
+ *   inspfs10, inspfs20 (totpoints, numspf) in;
+ *   twoematel(numspf, numspf, numspf, lowspf:highspf) out;
+ *   twoereduced(totpoints, numspf, lowspf:highspf) out.
+ * When inspfs10 == inspfs20 (the all_matel call, S/matel.f90:759) each
+ * unordered pair is convolved once and the mirror filled by symmetry. */
+int sdv_call_twoe_matel00(sdv_ctx* ctx, int lowspf, int highspf, const double* inspfs10,
+                          const double* inspfs20, double* twoematel, double* twoereduced);
+int sdv_call_twoe_matel00_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs10,
+                              const void* inspfs20, void* twoematel, void* twoereduced, void* stream);
+
+/* mult_ke (SD/coreproject.f90:2104-2116): in/out (totpoints, howmany). */
+int sdv_mult_ke(sdv_ctx* ctx, const double* in, double* out, int howmany);
+int sdv_mult_ke_dev(sdv_ctx* ctx, const void* in, void* out, int howmany, void* stream);
+
+/* mult_reducedpot (SD/coreproject.f90:1200-1205):
+ *   inspfs(totpoints, numspf), reducedpot(totpoints, numspf, firstspf:lastspf) in;
+ *   outspfs(totpoints, firstspf:lastspf) out. */
+int sdv_mult_reducedpot(sdv_ctx* ctx, int firstspf, int lastspf, const double* inspfs, double* outspfs,
+                        const double* reducedpot);
+int sdv_mult_reducedpot_dev(sdv_ctx* ctx, int firstspf, int lastspf, const void* inspfs, void* outspfs,
+                            const void* reducedpot, void* stream);
+
+/* get_reducedpot0 ZGEMM (S/mean.f90:556-558):
+ *   reducedpot(totpoints, numspf^2) = twoereduced(totpoints, numspf^2) . twoden(numspf^2, numspf^2) */
+int sdv_get_reducedpot(sdv_ctx* ctx, const double* twoereduced, const double* twoden, double* reducedpot);
+int sdv_get_reducedpot_dev(sdv_ctx* ctx, const void* twoereduced, const void* twoden, void* reducedpot,
+                           void* stream);
+
+/* Inspection (tests): the kernel threed_two on its first octant, n^3 reals
+ * K(i,j,k) for i,j,k in [0,n) (SD/sincproject.f90:182-187); the 1-D KE
+ * stencil ketot(1-n:n-1) (SD/SineDVR.f90:32); the circulant edge M in use. */
+int sdv_get_kernel_octant(sdv_ctx* ctx, double* koct);
+int sdv_get_ke_stencil(sdv_ctx* ctx, double* ketot);
+int sdv_circulant_edge(sdv_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SINCDVR_H */

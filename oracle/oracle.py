@@ -1,0 +1,232 @@
+"""ctypes wrapper over the CPU oracle (liborc.so) and the reference pieces
+compiled into oracle/_ref (libref_sinc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product library.  Arrays are numpy
+complex128 in Fortran order (first index fastest), exactly the layout of the
+reference's DATATYPE arrays.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liborc.so")
+REF_SO = os.path.join(HERE, "_ref", "libref_sinc.so")
+
+_lib = None
+_ref = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build(ref=True):
+    """Compile the oracle (and oracle/_ref when the reference is mounted)."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "oracle"])
+    if ref and os.path.isdir("/root/reference/MCTDH.SRC"):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build(ref=False)
+        _lib = ctypes.CDLL(ORACLE_SO)
+    return _lib
+
+
+def ref_lib():
+    """The reference's own SineDVR + DFFTPACK, or None if not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        _ref = ctypes.CDLL(REF_SO)
+    return _ref
+
+
+def _p(a):
+    assert a.flags["F_CONTIGUOUS"] or a.ndim <= 1
+    return a.ctypes.data_as(_dp)
+
+
+def set_threads(n):
+    lib().orc_set_threads(ctypes.c_int(n))
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def sinedvr(n, spacing):
+    ke = np.zeros(2 * n - 1)
+    fd = np.zeros(2 * n - 1)
+    pts = np.zeros(n)
+    lib().orc_sinedvr(ctypes.c_int(n), ctypes.c_double(spacing), _p(ke), _p(fd), _p(pts))
+    return ke, fd, pts
+
+
+def ref_sinedvr(n, spacing):
+    r = ref_lib()
+    if r is None:
+        return None
+    ke = np.zeros(2 * n - 1)
+    fd = np.zeros(2 * n - 1)
+    pts = np.zeros(n)
+    r.sinedvr_(_p(ke), _p(fd), _p(pts), ctypes.byref(ctypes.c_int(n)), ctypes.byref(ctypes.c_double(spacing)))
+    return ke, fd, pts
+
+
+def ecs_point(x, theta, stretch, dist, smooth):
+    out = [np.zeros(2) for _ in range(4)]
+    lib().orc_ecs_point(ctypes.c_double(x), ctypes.c_double(theta), ctypes.c_double(stretch),
+                        ctypes.c_double(dist), ctypes.c_double(smooth), *[_p(o) for o in out])
+    return [complex(o[0], o[1]) for o in out]  # F, J, J', J''
+
+
+def ecs_arrays(n, spacing, theta, stretch, dist, smooth):
+    ij = np.zeros((n ** 3, 3), dtype=np.complex128, order="F")
+    sd = np.zeros(n ** 3, dtype=np.complex128)
+    lib().orc_ecs_arrays(ctypes.c_int(n), ctypes.c_double(spacing), ctypes.c_double(theta),
+                         ctypes.c_double(stretch), ctypes.c_double(dist), ctypes.c_double(smooth),
+                         _p(ij), _p(sd))
+    return ij, sd
+
+
+def getinverse(N, nbig, nsmall, delta, want_tr=False):
+    L = 2 * N + 1
+    t = np.zeros((L, L, L), order="F")
+    ns = 2 * nsmall + 1
+    tr = np.zeros((ns, ns, ns), order="F") if want_tr else None
+    q = np.zeros((ns, ns, ns), order="F") if want_tr else None
+    rc = lib().orc_getinverse(ctypes.c_int(N), ctypes.c_int(nbig), ctypes.c_int(nsmall), ctypes.c_double(delta),
+                              _p(t), _p(tr) if want_tr else None, _p(q) if want_tr else None)
+    if rc:
+        raise ValueError(f"orc_getinverse rc={rc}")
+    return (t, tr, q) if want_tr else t
+
+
+def coulomb_kernel(n, spacing, nbig=240, nsmall=40):
+    L = 2 * n - 1
+    k = np.zeros((L, L, L), order="F")
+    rc = lib().orc_coulomb_kernel(ctypes.c_int(n), ctypes.c_double(spacing), ctypes.c_int(nbig),
+                                  ctypes.c_int(nsmall), _p(k))
+    if rc:
+        raise ValueError(f"orc_coulomb_kernel rc={rc}")
+    return k
+
+
+def fft1d(x):
+    """Forward unnormalised FFT along axis 0 of a (M, howmany) complex array."""
+    x = np.asfortranarray(x, dtype=np.complex128)
+    if x.ndim == 1:
+        x = x.reshape(-1, 1, order="F")
+    out = np.zeros_like(x, order="F")
+    lib().orc_fft1d(ctypes.c_int(x.shape[0]), ctypes.c_int(x.shape[1]), _p(x), _p(out))
+    return out
+
+
+def ref_zfftf(x):
+    """DFFTPACK zffti+zfftf per line, as CP/ftcore.f90:140-157 calls it."""
+    r = ref_lib()
+    if r is None:
+        return None
+    x = np.asfortranarray(x, dtype=np.complex128)
+    if x.ndim == 1:
+        x = x.reshape(-1, 1, order="F")
+    M = x.shape[0]
+    out = x.copy(order="F")
+    wsave = np.zeros(2 * (4 * M + 15))
+    for h in range(x.shape[1]):
+        line = np.ascontiguousarray(out[:, h])
+        r.zffti_(ctypes.byref(ctypes.c_int(M)), _p(wsave))
+        r.zfftf_(ctypes.byref(ctypes.c_int(M)), _p(line), _p(wsave))
+        out[:, h] = line
+    return out
+
+
+def op_tinv(n, kernel, den, M=None, circsize=None):
+    """den: (n^3, allsize) complex -> twoereduced (n^3, allsize)."""
+    den = np.asfortranarray(den, dtype=np.complex128)
+    if den.ndim == 1:
+        den = den.reshape(-1, 1, order="F")
+    allsize = den.shape[1]
+    M = 2 * n if M is None else M
+    circsize = allsize if circsize is None else circsize
+    red = np.zeros_like(den, order="F")
+    rc = lib().orc_op_tinv(ctypes.c_int(n), ctypes.c_int(M), _p(np.asfortranarray(kernel)), _p(den), _p(red),
+                           ctypes.c_int(allsize), ctypes.c_int(circsize))
+    if rc:
+        raise ValueError(f"orc_op_tinv rc={rc}")
+    return red
+
+
+def toeplitz_direct(n, kernel, den):
+    den = np.asfortranarray(den, dtype=np.complex128)
+    if den.ndim == 1:
+        den = den.reshape(-1, 1, order="F")
+    red = np.zeros_like(den, order="F")
+    lib().orc_toeplitz_direct(ctypes.c_int(n), _p(np.asfortranarray(kernel)), _p(den), _p(red),
+                              ctypes.c_int(den.shape[1]))
+    return red
+
+
+def twoe_matel(n, kernel, phi1, phi2, cnorm=False, low=1, high=None, M=None, notwo=False):
+    """Returns (twoematel(N,N,N,low:high), twoereduced(n^3,N,low:high))."""
+    phi1 = np.asfortranarray(phi1, dtype=np.complex128)
+    phi2 = np.asfortranarray(phi2, dtype=np.complex128)
+    N = phi1.shape[1]
+    high = N if high is None else high
+    nnn = high - low + 1
+    M = 2 * n if M is None else M
+    tm = np.zeros((N, N, N, nnn), dtype=np.complex128, order="F")
+    tr = np.zeros((n ** 3, N, nnn), dtype=np.complex128, order="F")
+    rc = lib().orc_twoe_matel(ctypes.c_int(n), ctypes.c_int(M), _p(np.asfortranarray(kernel)), ctypes.c_int(N),
+                              ctypes.c_int(low), ctypes.c_int(high), _p(phi1), _p(phi2), ctypes.c_int(int(cnorm)),
+                              ctypes.c_int(int(notwo)), _p(tm), _p(tr))
+    if rc:
+        raise ValueError(f"orc_twoe_matel rc={rc}")
+    return tm, tr
+
+
+def mult_ke(n, ke, inp, invjac=None, scalediag=None):
+    inp = np.asfortranarray(inp, dtype=np.complex128)
+    if inp.ndim == 1:
+        inp = inp.reshape(-1, 1, order="F")
+    out = np.zeros_like(inp, order="F")
+    lib().orc_mult_ke(ctypes.c_int(n), _p(np.ascontiguousarray(ke)),
+                      _p(np.asfortranarray(invjac)) if invjac is not None else None,
+                      _p(np.ascontiguousarray(scalediag)) if scalediag is not None else None,
+                      _p(inp), _p(out), ctypes.c_int(inp.shape[1]))
+    return out
+
+
+def mult_ke_dense(n, ke, inp, invjac=None, scalediag=None):
+    inp = np.ascontiguousarray(inp, dtype=np.complex128).ravel()
+    out = np.zeros_like(inp)
+    lib().orc_mult_ke_dense(ctypes.c_int(n), _p(np.ascontiguousarray(ke)),
+                            _p(np.asfortranarray(invjac)) if invjac is not None else None,
+                            _p(np.ascontiguousarray(scalediag)) if scalediag is not None else None,
+                            _p(inp), _p(out))
+    return out
+
+
+def mult_reducedpot(first, last, inspfs, reducedpot):
+    inspfs = np.asfortranarray(inspfs, dtype=np.complex128)
+    n3, N = inspfs.shape
+    rp = np.asfortranarray(reducedpot, dtype=np.complex128)
+    out = np.zeros((n3, last - first + 1), dtype=np.complex128, order="F")
+    lib().orc_mult_reducedpot(ctypes.c_int(n3), ctypes.c_int(N), ctypes.c_int(first), ctypes.c_int(last),
+                              _p(inspfs), _p(out), _p(rp))
+    return out
+
+
+def reducedpot(twoereduced, twoden):
+    tr = np.asfortranarray(twoereduced, dtype=np.complex128)
+    n3 = tr.shape[0]
+    N = int(round((tr.size // n3) ** 0.5))
+    g = np.asfortranarray(twoden, dtype=np.complex128)
+    out = np.zeros((n3, N, N), dtype=np.complex128, order="F")
+    lib().orc_reducedpot(ctypes.c_int(n3), ctypes.c_int(N), _p(tr), _p(g), _p(out))
+    return out

@@ -1,0 +1,172 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle on the
+same seeded inputs.  FP64 form: matrix elements within 1e-10 relative (the
+tolerance the task states for the plain drop-in); index bookkeeping exact.
+Mixed form (FP32 butterflies): its error is measured and bounded here."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def c0(orc):
+    from sincdvr import inputs
+    cfg = inputs.CONFIGS["c0_21"]
+    kern = orc.coulomb_kernel(cfg["n"], cfg["spacing"])
+    phi_h = inputs.gaussian_orbitals(cfg["n"], cfg["spacing"], cfg["norb"], cfg["ngauss"], cfg["seed"], cnorm=False)
+    phi_c = inputs.gaussian_orbitals(cfg["n"], cfg["spacing"], cfg["norb"], cfg["ngauss"], cfg["seed"], cnorm=True)
+    return dict(cfg=cfg, kern=kern, phi_h=phi_h, phi_c=phi_c)
+
+
+def test_kernel_and_stencil_parity(orc, torch_dev, c0):
+    from sincdvr import SincDVR
+    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    say("kernel octant n=%d" % n)
+    s = SincDVR(n, dx, 4)
+    ko = s.kernel_octant()
+    ref = c0["kern"][n - 1:, n - 1:, n - 1:]
+    assert relerr(ko, ref) < 1e-13
+    ke, _, _ = orc.sinedvr(n, dx)
+    assert relerr(s.ke_stencil(), ke) < 1e-15
+    assert s.circulant_edge >= 2 * n - 1
+
+
+@pytest.mark.parametrize("cnorm", [False, True])
+def test_twoe_matel_c0_fp64(orc, torch_dev, c0, cnorm):
+    from sincdvr import SincDVR
+    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    phi = c0["phi_c"] if cnorm else c0["phi_h"]
+    say("call_twoe_matel00 n=%d N=4 cnorm=%s host pointers" % (n, cnorm))
+    s = SincDVR(n, dx, 4, cnorm=cnorm)
+    tm, tr = s.call_twoe_matel00(1, 4, phi, phi)
+    otm, otr = orc.twoe_matel(n, c0["kern"], phi, phi, cnorm=cnorm)     # reference layout M = 2n
+    assert relerr(tr, otr) < TOL64
+    assert relerr(tm, otm) < TOL64
+    # device pointers give the same numbers
+    say("call_twoe_matel00 device pointers")
+    torch = torch_dev
+    p = torch.from_numpy(np.ascontiguousarray(phi.T)).cuda()
+    dtm, dtr = s.call_twoe_matel00(1, 4, p, p)
+    torch.cuda.synchronize()
+    assert np.array_equal(dtm.cpu().numpy().transpose(3, 2, 1, 0), tm)
+    assert np.array_equal(dtr.cpu().numpy().transpose(2, 1, 0), tr)
+
+
+@pytest.mark.parametrize("lowhigh", [(2, 3), (4, 4), (1, 1)])
+@pytest.mark.parametrize("same", [True, False])
+def test_twoe_matel_ranges_and_general_pairs(orc, torch_dev, c0, lowhigh, same):
+    from sincdvr import SincDVR
+    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    phi1 = c0["phi_h"]
+    phi2 = phi1 if same else np.asfortranarray(phi1[:, ::-1] * (0.5 + 0.25j))
+    low, high = lowhigh
+    say("call_twoe_matel00 low=%d high=%d same=%s" % (low, high, same))
+    s = SincDVR(n, dx, 4)
+    tm, tr = s.call_twoe_matel00(low, high, phi1, phi2)
+    otm, otr = orc.twoe_matel(n, c0["kern"], phi1, phi2, low=low, high=high)
+    assert tm.shape == otm.shape and tr.shape == otr.shape
+    assert relerr(tr, otr) < TOL64
+    assert relerr(tm, otm) < TOL64
+
+
+def test_flags_notwo_and_debug(orc, torch_dev, c0):
+    from sincdvr import SincDVR
+    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    say("notwoflag / debugflag=4242")
+    tm, tr = SincDVR(n, dx, 4, notwoflag=1).call_twoe_matel00(1, 4, c0["phi_h"], c0["phi_h"])
+    assert not tm.any() and not tr.any()
+    den = np.asfortranarray(c0["phi_h"] * 1.0)
+    assert not SincDVR(n, dx, 4, debugflag=4242).op_tinv(den).any()
+
+
+def test_op_tinv_size_error(torch_dev, c0):
+    from sincdvr import SdvError, SincDVR
+    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    s = SincDVR(n, dx, 4)
+    with pytest.raises(SdvError):
+        s.op_tinv(np.zeros((n ** 3, 3), dtype=complex, order="F"), circsize=2)
+    with pytest.raises(SdvError):
+        s.call_twoe_matel00(0, 4, c0["phi_h"], c0["phi_h"])
+
+
+@pytest.mark.parametrize("n", [5, 9, 12, 16, 21, 25, 27, 32, 36, 45, 48, 54, 63, 64, 72, 81])
+def test_op_tinv_all_circulant_sizes(orc, torch_dev, n):
+    """Every compiled (P, Q) factorisation against the oracle at M = 2n."""
+    from sincdvr import SincDVR
+    dx = 0.3
+    rng = np.random.default_rng(n)
+    den = np.asfortranarray(rng.standard_normal((n ** 3, 2)) + 1j * rng.standard_normal((n ** 3, 2)))
+    say("op_tinv n=%d" % n)
+    s = SincDVR(n, dx, 2)
+    kern = orc.coulomb_kernel(n, dx)
+    ref = orc.op_tinv(n, kern, den)
+    assert relerr(s.op_tinv(den), ref) < 1e-12
+    s.set_precision(1)
+    e32 = relerr(s.op_tinv(den), ref)
+    say("op_tinv n=%d fp32 rel err %.3e" % (n, e32))
+    assert e32 < 2e-5
+
+
+@pytest.mark.parametrize("ecs", [False, True])
+def test_mult_ke(orc, torch_dev, ecs):
+    from sincdvr import SincDVR, inputs
+    n, dx = 41, 0.3
+    phi = inputs.gaussian_orbitals(n, dx, 3, 4, seed=11)
+    sc = inputs.ECS_41 if ecs else None
+    say("mult_ke n=%d ecs=%s" % (n, ecs))
+    s = SincDVR(n, dx, 3, scaling=sc)
+    out = s.mult_ke(phi)
+    ke, _, _ = orc.sinedvr(n, dx)
+    ij = sd = None
+    if ecs:
+        ij, sd = orc.ecs_arrays(n, dx, sc["theta"], sc["stretch"], sc["dist"], sc["smooth"])
+    ref = orc.mult_ke(n, ke, phi, ij, sd)
+    assert relerr(out, ref) < 1e-12
+
+
+def test_mean_field(orc, torch_dev, c0):
+    from sincdvr import SincDVR
+    n, dx, N = c0["cfg"]["n"], c0["cfg"]["spacing"], 4
+    rng = np.random.default_rng(9)
+    tr = np.asfortranarray(rng.standard_normal((n ** 3, N, N)) + 1j * rng.standard_normal((n ** 3, N, N)))
+    g = np.asfortranarray(rng.standard_normal((N, N, N, N)) + 1j * rng.standard_normal((N, N, N, N)))
+    say("get_reducedpot / mult_reducedpot")
+    s = SincDVR(n, dx, N)
+    rp = s.get_reducedpot(tr, g)
+    assert relerr(rp, orc.reducedpot(tr, g)) < 1e-13
+    out = s.mult_reducedpot(2, 4, c0["phi_h"], np.asfortranarray(rp[:, :, 1:4]))
+    assert relerr(out, orc.mult_reducedpot(2, 4, c0["phi_h"], rp[:, :, 1:4])) < 1e-13
+
+
+def test_twoe_matel_c1_41(orc, torch_dev):
+    """41^3 grid, 10 orbitals (55 unique / 100 ordered pairs), hermnorm."""
+    from sincdvr import SincDVR, inputs
+    cfg = inputs.CONFIGS["c1_41"]
+    n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
+    phi = inputs.gaussian_orbitals(n, dx, N, cfg["ngauss"], cfg["seed"])
+    say("call_twoe_matel00 n=41 N=10")
+    s = SincDVR(n, dx, N)
+    tm, tr = s.call_twoe_matel00(1, N, phi, phi)
+    kern = orc.coulomb_kernel(n, dx)
+    otm, otr = orc.twoe_matel(n, kern, phi, phi, M=81)
+    assert relerr(tr, otr) < TOL64
+    assert relerr(tm, otm) < TOL64

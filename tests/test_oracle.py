@@ -1,0 +1,281 @@
+"""Pins the CPU oracle (oracle/sincdvr_oracle.c) without trusting it:
+closed forms, brute force on tiny grids, invariants of the defining linear
+systems, an independent FFT library (numpy's pocketfft), the reference's own
+SineDVR / DFFTPACK compiled into oracle/_ref, and the committed fixtures that
+the reference's compiled pieces produced (tests/golden/)."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- a1 sineDVR
+def test_sinedvr_closed_form(orc):
+    for n, dx in [(5, 0.4), (21, 0.4), (41, 0.3)]:
+        ke, fd, pts = orc.sinedvr(n, dx)
+        i = np.arange(1 - n, n)
+        with np.errstate(divide="ignore"):
+            ref = np.where(i == 0, np.pi ** 2 / 3, 2.0 * (-1.0) ** i / np.where(i == 0, 1, i) ** 2) / 2 / dx ** 2
+        assert relerr(ke, ref) < 1e-15
+        assert np.all(ke == ke[::-1])                        # symmetric Toeplitz stencil
+        assert abs(pts[0] + pts[-1]) < 1e-14 and abs(pts[1] - pts[0] - dx) < 1e-14
+        # symbol of the stencil is k^2/2 on the band (sum over all offsets,
+        # infinite-grid limit approximated by a long stencil)
+    ke, _, _ = orc.sinedvr(2001, 0.3)
+    m = np.arange(-2000, 2001)
+    for k in (0.5, 2.0, 5.0):
+        sym = np.sum(ke * np.cos(k * m * 0.3))
+        assert abs(sym - k * k / 2) < 2e-3 * max(1, k * k)
+
+
+def test_sinedvr_matches_reference_build(orc):
+    r = orc.ref_sinedvr(21, 0.4)
+    if r is None:
+        pytest.skip("oracle/_ref not built (reference sources not mounted)")
+    for a, b in zip(orc.sinedvr(21, 0.4), r):
+        assert np.array_equal(a, b)
+
+
+def test_sinedvr_golden(orc):
+    g = np.load(os.path.join(GOLD, "ref_sinedvr.npz"))
+    for key in g.files:
+        n, dx = key.split("_")[1:]
+        ke, fd, pts = orc.sinedvr(int(n), float(dx))
+        assert np.array_equal(np.concatenate([ke, fd, pts]), g[key])
+
+
+# ------------------------------------------------------------ a8/a9 1-D FFT
+@pytest.mark.parametrize("M", [1, 2, 7, 42, 45, 64, 82, 90, 162])
+def test_fft_vs_numpy(orc, M):
+    rng = np.random.default_rng(M)
+    x = rng.standard_normal((M, 3)) + 1j * rng.standard_normal((M, 3))
+    assert relerr(orc.fft1d(x), np.fft.fft(x, axis=0)) < 2e-15 * max(1, np.log2(M))
+
+
+def test_fft_golden_dfftpack(orc):
+    """Vectors produced by the reference's DFFTPACK zffti+zfftf (compiled from
+    S/DFFTPACK into oracle/_ref by tests/golden/make_golden.py)."""
+    g = np.load(os.path.join(GOLD, "ref_zfftf.npz"))
+    for key in [k for k in g.files if k.startswith("in_")]:
+        M = int(key.split("_")[1])
+        x = g[key]
+        assert relerr(orc.fft1d(x), g["out_%d" % M]) < 2e-15 * max(1, np.log2(M))
+
+
+# ----------------------------------------------------------- a3 GETINVERSE
+def _t1d(ns, d):
+    i = np.arange(ns)
+    D = i[:, None] - i[None, :]
+    with np.errstate(divide="ignore"):
+        T = np.where(D == 0, np.pi ** 2 / (6 * d * d), (-1.0) ** D / (d * d * np.where(D == 0, 1, D) ** 2))
+    return T
+
+
+@pytest.fixture(scope="module")
+def tooth_small(orc):
+    # N = 5, nbig = 60, nsmall = 8 keeps this fast while exercising every branch
+    return orc.getinverse(5, 60, 8, 0.4, want_tr=True)
+
+
+def test_getinverse_solves_linear_system(tooth_small):
+    _, TR, Q = tooth_small
+    T = _t1d(TR.shape[0], 0.4)
+    res = np.einsum("ai,ijk->ajk", T, Q) + np.einsum("bj,ijk->ibk", T, Q) + np.einsum("ck,ijk->ijc", T, Q)
+    assert relerr(res, TR) < 1e-13
+
+
+def test_getinverse_tr_brute_force(tooth_small):
+    """TR(i,j,k) = -sum_p T(i-p)R(p,j,k)+T(j-p)R(i,p,k)+T(k-p)R(i,j,p) (+2 pi at 0)."""
+    _, TR, _ = tooth_small
+    d, nbig, ns = 0.4, 60, 8
+
+    def R(i, j, k):
+        return 0.0 if i == j == k == 0 else d ** 3 / np.sqrt((i * d) ** 2 + (j * d) ** 2 + (k * d) ** 2)
+
+    def T(m):
+        return np.pi ** 2 / (6 * d * d) if m == 0 else (-1.0) ** m / (d * d * m * m)
+
+    for (i, j, k) in [(0, 0, 0), (1, 0, 0), (2, -3, 1), (-8, 8, 5), (0, 7, -2)]:
+        s = sum(T(i - p) * R(p, j, k) + T(j - p) * R(i, p, k) + T(k - p) * R(i, j, p) for p in range(-nbig, nbig + 1))
+        ref = -s + (2 * np.pi if i == j == k == 0 else 0.0)
+        assert abs(TR[i + ns, j + ns, k + ns] - ref) < 1e-12 * max(1, abs(ref))
+
+
+def test_getinverse_far_field_and_symmetry(orc):
+    N, d = 12, 0.3
+    t = orc.getinverse(N, 60, 8, d)
+    idx = np.arange(-N, N + 1)
+    I, J, K = np.meshgrid(idx, idx, idx, indexing="ij")
+    far = np.maximum(np.maximum(abs(I), abs(J)), abs(K)) > 8
+    r = np.sqrt((I * d) ** 2 + (J * d) ** 2 + (K * d) ** 2)
+    assert np.allclose(t[far], d ** 3 / r[far], rtol=1e-15, atol=0)
+    # reflections and axis permutations of the cube
+    for op in (lambda a: a[::-1], lambda a: a[:, ::-1], lambda a: a.transpose(1, 0, 2), lambda a: a.transpose(2, 1, 0)):
+        assert relerr(op(t), t) < 1e-13
+
+
+def test_coulomb_kernel_is_inverse_laplacian(orc):
+    """Near the origin the corrected kernel satisfies the sinc-DVR Poisson
+    equation T3d K = 2 pi delta / D^3 (up to the nbig truncation)."""
+    n, d = 7, 0.4
+    k = orc.coulomb_kernel(n, d, nbig=100, nsmall=12)
+    kd = k * d ** 3
+    ns = 2 * n - 1
+    T = _t1d(ns, d)
+    lap = np.einsum("ai,ijk->ajk", T, kd) + np.einsum("bj,ijk->ibk", T, kd) + np.einsum("ck,ijk->ijc", T, kd)
+    c = n - 1
+    # only the centre is free of truncation-boundary effects
+    assert abs(lap[c, c, c] - 2 * np.pi) < 0.05 * 2 * np.pi
+    assert np.abs(lap[c - 1:c + 2, c - 1:c + 2, c - 1:c + 2]).sum() - abs(lap[c, c, c]) < 0.2
+
+
+# ------------------------------------------------------- a5/a6/a7 op_tinv
+@pytest.mark.parametrize("n", [3, 5, 6])
+def test_op_tinv_brute_force(orc, n):
+    rng = np.random.default_rng(n)
+    kern = orc.coulomb_kernel(n, 0.4, nbig=40, nsmall=8)
+    den = rng.standard_normal((n ** 3, 3)) + 1j * rng.standard_normal((n ** 3, 3))
+    direct = orc.toeplitz_direct(n, kern, den)
+    assert relerr(orc.op_tinv(n, kern, den), direct) < 1e-14            # reference layout M = 2n
+    for M in (2 * n - 1, 2 * n + 3, 3 * n):                               # any M >= 2n-1
+        assert relerr(orc.op_tinv(n, kern, den, M=M), direct) < 1e-14
+    assert relerr(orc.op_tinv(n, kern, den, circsize=1), direct) < 1e-14
+
+
+def test_op_tinv_delta_linearity_conjugation(orc):
+    n = 5
+    kern = orc.coulomb_kernel(n, 0.4, nbig=40, nsmall=8)
+    L = 2 * n - 1
+    for (a, b, c) in [(0, 0, 0), (4, 1, 2), (2, 2, 2)]:
+        den = np.zeros(n ** 3, dtype=complex)
+        den[a + n * (b + n * c)] = 1.0
+        v = orc.op_tinv(n, kern, den)[:, 0].reshape((n, n, n), order="F")
+        ref = kern[(n - 1 - a):(2 * n - 1 - a), (n - 1 - b):(2 * n - 1 - b), (n - 1 - c):(2 * n - 1 - c)]
+        assert ref.shape == (n, n, n) and L == kern.shape[0]
+        assert relerr(v, ref) < 1e-14
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(n ** 3) + 1j * rng.standard_normal(n ** 3)
+    y = rng.standard_normal(n ** 3) + 1j * rng.standard_normal(n ** 3)
+    vx, vy = orc.op_tinv(n, kern, x), orc.op_tinv(n, kern, y)
+    assert relerr(orc.op_tinv(n, kern, 2 * x - 3j * y), 2 * vx - 3j * vy) < 1e-14
+    assert relerr(orc.op_tinv(n, kern, np.conj(x)), np.conj(vx)) < 1e-14
+
+
+def test_op_tinv_size_error(orc):
+    with pytest.raises(ValueError):
+        orc.op_tinv(3, np.zeros((5, 5, 5)), np.zeros((27, 3), dtype=complex), circsize=2)
+
+
+# ------------------------------------------------------ a10 twoe matel
+def _brute_twoe(orc, n, kern, phi1, phi2, cnorm, low, high):
+    N = phi1.shape[1]
+    cj = (lambda z: z) if cnorm else np.conj
+    tr = np.zeros((n ** 3, N, high - low + 1), dtype=complex, order="F")
+    for b2 in range(low, high + 1):
+        for a2 in range(1, N + 1):
+            den = phi1[:, a2 - 1] * cj(phi2[:, b2 - 1])
+            tr[:, a2 - 1, b2 - low] = orc.toeplitz_direct(n, kern, den)[:, 0]
+    tm = np.zeros((N, N, N, high - low + 1), dtype=complex, order="F")
+    for b1 in range(N):
+        for a1 in range(N):
+            myden = phi1[:, a1] * cj(phi2[:, b1])
+            for b2 in range(high - low + 1):
+                for a2 in range(N):
+                    tm[a1, b1, a2, b2] = np.sum(tr[:, a2, b2] * myden)
+    return tm, tr
+
+
+@pytest.mark.parametrize("cnorm", [False, True])
+@pytest.mark.parametrize("lowhigh", [(1, 3), (2, 3), (2, 2)])
+def test_twoe_matel_bookkeeping(orc, cnorm, lowhigh):
+    n, N = 4, 3
+    rng = np.random.default_rng(7)
+    kern = orc.coulomb_kernel(n, 0.5, nbig=30, nsmall=6)
+    phi1 = np.asfortranarray(rng.standard_normal((n ** 3, N)) + 1j * rng.standard_normal((n ** 3, N)))
+    phi2 = np.asfortranarray(rng.standard_normal((n ** 3, N)) + 1j * rng.standard_normal((n ** 3, N)))
+    low, high = lowhigh
+    for p2 in (phi1, phi2):
+        tm, tr = orc.twoe_matel(n, kern, phi1, p2, cnorm=cnorm, low=low, high=high)
+        btm, btr = _brute_twoe(orc, n, kern, phi1, p2, cnorm, low, high)
+        assert relerr(tr, btr) < 1e-13
+        assert relerr(tm, btm) < 1e-13
+
+
+def test_twoe_matel_symmetries(orc):
+    n, N = 4, 3
+    rng = np.random.default_rng(3)
+    kern = orc.coulomb_kernel(n, 0.5, nbig=30, nsmall=6)
+    phi = np.asfortranarray(rng.standard_normal((n ** 3, N)) + 1j * rng.standard_normal((n ** 3, N)))
+    tm, tr = orc.twoe_matel(n, kern, phi, phi, cnorm=False)
+    assert relerr(tr.transpose(0, 2, 1), np.conj(tr)) < 1e-14           # hermnorm V_ba = conj V_ab
+    assert relerr(tm, tm.transpose(2, 3, 0, 1)) < 1e-13                  # Coulomb symmetry
+    tmc, trc = orc.twoe_matel(n, kern, phi, phi, cnorm=True)
+    assert np.array_equal(trc, trc.transpose(0, 2, 1))                   # cnorm V_ab == V_ba bitwise
+    z, _ = orc.twoe_matel(n, kern, phi, phi, notwo=True)
+    assert not z.any()
+
+
+# --------------------------------------------------------------- a11 KE
+@pytest.mark.parametrize("ecs", [False, True])
+def test_mult_ke_dense(orc, ecs):
+    n, d = 5, 0.4
+    ke, _, _ = orc.sinedvr(n, d)
+    ij = sd = None
+    if ecs:
+        ij, sd = orc.ecs_arrays(n, d, 0.3, 1.0, 0.3, 0.45)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(n ** 3) + 1j * rng.standard_normal(n ** 3)
+    a = orc.mult_ke(n, ke, x, ij, sd)[:, 0]
+    b = orc.mult_ke_dense(n, ke, x, ij, sd)
+    assert relerr(a, b) < 1e-14
+
+
+def test_mult_ke_wave_packet(orc):
+    """The sinc-DVR KE is spectrally exact for a well-resolved wave packet
+    f = exp(i k.r - a r^2):  -1/2 lap f = -1/2 sum_d [(i k_d - 2 a x_d)^2 - 2a] f."""
+    n, d, al = 41, 0.3, 0.5
+    ke, _, pts = orc.sinedvr(n, d)
+    k = np.array([0.7, -1.1, 0.4])
+    X, Y, Z = np.meshgrid(pts, pts, pts, indexing="ij")
+    f = np.exp(1j * (k[0] * X + k[1] * Y + k[2] * Z) - al * (X * X + Y * Y + Z * Z))
+    lap = sum((1j * kd - 2 * al * xd) ** 2 - 2 * al for kd, xd in zip(k, (X, Y, Z))) * f
+    out = orc.mult_ke(n, ke, f.ravel(order="F"))[:, 0].reshape((n, n, n), order="F")
+    assert relerr(out, -0.5 * lap) < 1e-7
+
+
+def test_ecs_functions(orc):
+    th, st, R0, S = 0.3, 1.0, 3.75, 2.25
+    fac = np.exp(1j * th) * st
+    assert orc.ecs_point(1.0, th, st, R0, S)[1] == 1.0
+    assert abs(orc.ecs_point(6.5, th, st, R0, S)[1] - fac) < 1e-15
+    assert abs(orc.ecs_point(-6.0, th, st, R0, S)[1] - fac) < 1e-15      # |x| = R0 + S reaches fac
+    # J = F', J' = dJ/dx, J'' = d^2J/dx^2 inside the ramps (finite differences)
+    h = 1e-5
+    for x in (4.1, 5.0, -4.6, -5.7):
+        F0, J0, D0, DD0 = orc.ecs_point(x, th, st, R0, S)
+        Fp, Jp, Dp, _ = orc.ecs_point(x + h, th, st, R0, S)
+        Fm, Jm, Dm, _ = orc.ecs_point(x - h, th, st, R0, S)
+        assert abs((Fp - Fm) / (2 * h) - J0) < 1e-8
+        assert abs((Jp - Jm) / (2 * h) - D0) < 1e-8
+        assert abs((Dp - Dm) / (2 * h) - DD0) < 1e-7
+    ij, sd = orc.ecs_arrays(5, 0.4, th, st, 10.0, 1.0)                    # ramp outside the grid
+    assert np.all(ij == 1.0) and not sd.any()
+
+
+# ------------------------------------------------------ a12/a13 mean field
+def test_mean_field(orc):
+    rng = np.random.default_rng(5)
+    n3, N = 64, 3
+    tr = rng.standard_normal((n3, N, N)) + 1j * rng.standard_normal((n3, N, N))
+    g = rng.standard_normal((N, N, N, N)) + 1j * rng.standard_normal((N, N, N, N))
+    rp = orc.reducedpot(np.asfortranarray(tr), np.asfortranarray(g))
+    ref = tr.reshape(n3, N * N, order="F") @ g.reshape(N * N, N * N, order="F")
+    assert relerr(rp.reshape(n3, N * N, order="F"), ref) < 1e-14
+    phi = rng.standard_normal((n3, N)) + 1j * rng.standard_normal((n3, N))
+    out = orc.mult_reducedpot(2, 3, phi, rp[:, :, 1:3])
+    ref2 = np.einsum("xki,xk->xi", rp[:, :, 1:3], phi)
+    assert relerr(out, ref2) < 1e-14

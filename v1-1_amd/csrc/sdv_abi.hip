@@ -1,0 +1,509 @@
+// sdv_abi.hip -- C-ABI entry points and host-side orchestration.
+//
+// Mirrors the SINCDVR project interface of the reference (op_tinv,
+// call_twoe_matel00, mult_ke, mult_reducedpot, get_reducedpot0's GEMM) on
+// device-resident data.  No computation of the operators happens on the host:
+// the host builds index lists (pair bookkeeping) and launches kernels.
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sdv_internal.h"
+
+namespace sdv {
+
+static std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+// Per-axis tables: KE stencil (SD/SineDVR.f90:22-33), ECS 1/J and the
+// per-axis scalediag terms (SD/init_sinc_new.f90:5-166, 493-521).
+__device__ double fscaled_d(double x) {
+  const double s = sin(kPiD * x);
+  return (-1.0) / 12.0 / kPiD * s * s * s - 1.0 / 2.0 / kPiD * s + x / 2.0;
+}
+__device__ double jscaled_d(double x) {
+  const double c = cos(kPiD * x);
+  return 1.0 / 4.0 * c * c * c - 3.0 / 4.0 * c + 0.5;
+}
+__device__ double djscaled_d(double x) {
+  const double s = sin(kPiD * x);
+  return 0.75 * kPiD * s * s * s;
+}
+__device__ double ddjscaled_d(double x) {
+  const double s = sin(kPiD * x), c = cos(kPiD * x);
+  return 9.0 / 4.0 * kPiD * kPiD * s * s * c;
+}
+
+__global__ void k_init_tables(int n, double spacing, int ecs, double theta, double stretch, double R0, double S,
+                              double* ke, z64* invj, z64* sdiag) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 2 * n - 1) {
+    const int i = t - (n - 1);
+    double k;
+    if (i == 0)
+      k = kPiD * kPiD / 3.0;
+    else
+      k = 2.0 / ((double)i * (double)i) * ((i & 1) ? -1.0 : 1.0);
+    ke[t] = k / 2.0 / (spacing * spacing);
+  }
+  if (ecs && t < n) {
+    const double p1 = 1 * spacing, pn = n * spacing;
+    const double x = (t + 1) * spacing - (p1 + pn) / 2.0;
+    const z64 fac = mk(cos(theta) * stretch, sin(theta) * stretch);
+    const z64 fm1 = mk(fac.re - 1.0, fac.im);
+    z64 j, dj, ddj;
+    const z64 zero = mk(0.0, 0.0);
+    if (x <= -(R0 + S) || x >= R0 + S) {
+      j = fac;
+      dj = zero;
+      ddj = zero;
+    } else if (fabs(x) <= R0) {
+      j = mk(1.0, 0.0);
+      dj = zero;
+      ddj = zero;
+    } else if (x >= R0) {
+      const double xx = (x - R0) / S;
+      j = mk(1.0, 0.0) + scale(fm1, jscaled_d(xx));
+      dj = scale(fm1, djscaled_d(xx) / S);
+      ddj = scale(fm1, ddjscaled_d(xx) / (S * S));
+    } else {
+      const double xx = (-x - R0) / S;
+      j = mk(1.0, 0.0) + scale(fm1, jscaled_d(xx));
+      dj = scale(mk(-fm1.re, -fm1.im), djscaled_d(xx) / S);
+      ddj = scale(fm1, ddjscaled_d(xx) / (S * S));
+    }
+    const double d = j.re * j.re + j.im * j.im;
+    const z64 a = mk(j.re / d, -j.im / d);  // 1/J
+    const z64 a2 = a * a, a3 = a2 * a, a4 = a2 * a2;
+    const z64 t1 = scale(a4 * (dj * dj), 3.0 / 8.0);
+    const z64 t2 = scale(a3 * ddj, 1.0 / 4.0);
+    for (int d3 = 0; d3 < 3; ++d3) {
+      invj[d3 * n + t] = a;
+      sdiag[d3 * n + t] = t1 - t2;
+    }
+  }
+}
+
+// twoematel(1a,1b,2a,2b) from the four real products of (row r, col c).
+__global__ void k_assemble(int N, int nnn, const int2* __restrict__ rmap, const int2* __restrict__ cmap,
+                           const double* __restrict__ g, int C2, z64* __restrict__ out) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long tot = (long)N * N * N * nnn;
+  if (idx >= tot) return;
+  const int rr = (int)(idx % ((long)N * N));     // (1a, 1b)
+  const int cc = (int)(idx / ((long)N * N));     // (2a, 2b-low)
+  const int2 rm = rmap[rr], cm = cmap[cc];
+  const double* gr = g + (size_t)(2 * rm.x) * C2 + 2 * cm.x;
+  const double RR = gr[0], RI = gr[1], IR = gr[C2], II = gr[C2 + 1];
+  z64 v;
+  if (!rm.y && !cm.y)
+    v = mk(RR - II, RI + IR);          // sum rho V
+  else if (rm.y && !cm.y)
+    v = mk(RR + II, RI - IR);          // sum conj(rho) V
+  else if (!rm.y && cm.y)
+    v = mk(RR + II, IR - RI);          // sum rho conj(V)
+  else
+    v = mk(RR - II, -(RI + IR));       // sum conj(rho) conj(V)
+  out[idx] = v;
+}
+
+}  // namespace sdv
+
+using namespace sdv;
+
+struct sdv_ctx {
+  sdv_config cfg;
+  int n = 0, n3 = 0, N = 0;
+  int precision = 0;
+  int pair_batch = 8;
+  Spectrum spec;
+  double* ke = nullptr;
+  z64* invj = nullptr;
+  z64* sdiag = nullptr;
+  hipStream_t stream = nullptr;
+  // growable device buffers
+  struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+  };
+  Buf work, scratch, idx, gout, h_in1, h_in2, h_out1, h_out2;
+};
+
+static int ensure(sdv_ctx::Buf& b, size_t bytes) {
+  if (b.cap >= bytes) return SDV_OK;
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    b.p = nullptr;
+    set_error("device allocation of " + std::to_string(bytes) + " bytes failed");
+    return SDV_ERR_NOMEM;
+  }
+  b.cap = bytes;
+  return SDV_OK;
+}
+
+#define SDV_TRY(expr)          \
+  do {                         \
+    int _rc = (expr);          \
+    if (_rc != SDV_OK) return _rc; \
+  } while (0)
+
+extern "C" {
+
+void sdv_config_default(sdv_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->numpoints = 15;
+  c->spacing = 0.25;
+  c->toothnbig = 240;
+  c->toothnsmall = 40;
+  c->numspf = 1;
+  c->scalingstretch = 1.0;
+  c->scalingdistance = 10000.0;
+  c->smoothness = 5.0;
+}
+
+const char* sdv_last_error(void) { return last_error(); }
+
+int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
+  SDV_REQUIRE(cfg && out, SDV_ERR_ARG, "null argument");
+  SDV_REQUIRE(cfg->numpoints >= 2, SDV_ERR_ARG, "numpoints must be >= 2");
+  SDV_REQUIRE(cfg->spacing > 0, SDV_ERR_ARG, "spacing must be positive");
+  SDV_REQUIRE(cfg->numspf >= 1, SDV_ERR_ARG, "numspf must be >= 1");
+  SDV_REQUIRE(cfg->precision == 0 || cfg->precision == 1, SDV_ERR_ARG, "precision must be 0 or 1");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no HIP device visible");
+    return SDV_ERR_NODEVICE;
+  }
+  int dev = 0;
+  SDV_HIP_CHECK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  SDV_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error(std::string("library is built for gfx950, device is ") + prop.gcnArchName);
+    return SDV_ERR_NODEVICE;
+  }
+  sdv_ctx* c = new sdv_ctx();
+  c->cfg = *cfg;
+  c->n = cfg->numpoints;
+  c->n3 = c->n * c->n * c->n;
+  c->N = cfg->numspf;
+  c->precision = cfg->precision;
+  if (const char* pb = std::getenv("SDV_PAIR_BATCH")) c->pair_batch = std::max(1, std::atoi(pb));
+  int rc = SDV_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    delete c;
+    return SDV_ERR_HIP;
+  }
+  rc = build_spectrum(c->n, cfg->spacing, cfg->toothnbig, cfg->toothnsmall, &c->spec, c->stream);
+  if (rc == SDV_OK) {
+    const int n = c->n;
+    if (hipMalloc(&c->ke, (2 * n) * sizeof(double)) != hipSuccess) rc = SDV_ERR_NOMEM;
+    if (rc == SDV_OK && cfg->scalingflag) {
+      if (hipMalloc(&c->invj, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
+      if (rc == SDV_OK && hipMalloc(&c->sdiag, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
+    }
+    if (rc == SDV_OK) {
+      hipLaunchKernelGGL(k_init_tables, dim3((2 * n + 255) / 256), dim3(256), 0, c->stream, n, cfg->spacing,
+                         cfg->scalingflag ? 1 : 0, cfg->scalingtheta, cfg->scalingstretch, cfg->scalingdistance,
+                         cfg->smoothness, c->ke, c->invj, c->sdiag);
+      if (hipStreamSynchronize(c->stream) != hipSuccess) rc = SDV_ERR_HIP;
+    }
+    if (rc == SDV_ERR_NOMEM) set_error("device allocation failed in sdv_init");
+  }
+  if (rc != SDV_OK) {
+    sdv_free(c);
+    return rc;
+  }
+  *out = c;
+  return SDV_OK;
+}
+
+int sdv_free(sdv_ctx* c) {
+  if (!c) return SDV_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_spectrum(&c->spec);
+  hipFree(c->ke);
+  hipFree(c->invj);
+  hipFree(c->sdiag);
+  for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->idx, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2})
+    hipFree(b->p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return SDV_OK;
+}
+
+int sdv_set_precision(sdv_ctx* c, int p) {
+  SDV_REQUIRE(c, SDV_ERR_ARG, "null context");
+  SDV_REQUIRE(p == 0 || p == 1, SDV_ERR_ARG, "precision must be 0 or 1");
+  c->precision = p;
+  return SDV_OK;
+}
+
+int sdv_circulant_edge(sdv_ctx* c) { return c ? c->spec.g.M : SDV_ERR_ARG; }
+
+int sdv_get_kernel_octant(sdv_ctx* c, double* koct) {
+  SDV_REQUIRE(c && koct, SDV_ERR_ARG, "null argument");
+  SDV_HIP_CHECK(hipMemcpy(koct, c->spec.koct, (size_t)c->n3 * sizeof(double), hipMemcpyDeviceToHost));
+  return SDV_OK;
+}
+
+int sdv_get_ke_stencil(sdv_ctx* c, double* ketot) {
+  SDV_REQUIRE(c && ketot, SDV_ERR_ARG, "null argument");
+  SDV_HIP_CHECK(hipMemcpy(ketot, c->ke, (size_t)(2 * c->n - 1) * sizeof(double), hipMemcpyDeviceToHost));
+  return SDV_OK;
+}
+
+// Run the convolution over a list of pair items, in batches sized to the
+// work buffer.
+static int run_pairs(sdv_ctx* c, const std::vector<PairItem>& items, const void* phi1, const void* phi2,
+                     int density_mode, void* out, hipStream_t st) {
+  const int cn = c->cfg.cnormflag;
+  const int np = (int)items.size();
+  if (np == 0) return SDV_OK;
+  const int batch = std::min(np, c->pair_batch);
+  SDV_TRY(ensure(c->work, conv_work_bytes(c->spec.g, batch, c->precision)));
+  SDV_TRY(ensure(c->idx, (size_t)np * sizeof(PairItem)));
+  SDV_HIP_CHECK(hipMemcpyAsync(c->idx.p, items.data(), (size_t)np * sizeof(PairItem), hipMemcpyHostToDevice, st));
+  for (int p0 = 0; p0 < np; p0 += batch) {
+    ConvArgsHost a;
+    a.n = c->n;
+    a.npairs = std::min(batch, np - p0);
+    a.phi1 = phi1;
+    a.phi2 = phi2;
+    a.density_mode = density_mode;
+    a.conj_b = cn ? 0 : 1;
+    a.mirror_conj = cn ? 0 : 1;
+    a.pairs_dev = (const PairItem*)c->idx.p + p0;
+    a.work = c->work.p;
+    a.out = out;
+    SDV_TRY(conv_launch(c->spec, a, c->precision, st));
+  }
+  return SDV_OK;
+}
+
+int sdv_op_tinv_dev(sdv_ctx* c, const void* den, void* red, int allsize, int circsize, void* stream) {
+  SDV_REQUIRE(c && den && red, SDV_ERR_ARG, "null argument");
+  SDV_REQUIRE(allsize >= 0 && circsize > 0 && allsize % circsize == 0, SDV_ERR_ARG,
+              "SIZE ERROR OP_TINV: allsize must be a multiple of circsize");
+  hipStream_t st = (hipStream_t)stream;
+  const int dbg = c->cfg.debugflag;
+  if (dbg == 4040 || dbg == 4242 || dbg == 4343) {  // SD/coreproject.f90:260-263
+    SDV_HIP_CHECK(hipMemsetAsync(red, 0, (size_t)c->n3 * allsize * sizeof(z64), st));
+    return SDV_OK;
+  }
+  std::vector<PairItem> items(allsize);
+  for (int i = 0; i < allsize; ++i) items[i] = PairItem{i, 0, i, -1};
+  return run_pairs(c, items, den, den, 1, red, st);
+}
+
+int sdv_call_twoe_matel00_dev(sdv_ctx* c, int low, int high, const void* phi1, const void* phi2, void* twoematel,
+                              void* twoereduced, void* stream) {
+  SDV_REQUIRE(c && phi1 && phi2 && twoematel && twoereduced, SDV_ERR_ARG, "null argument");
+  const int N = c->N, n3 = c->n3;
+  const int nnn = high - low + 1;
+  if (nnn <= 0) return SDV_OK;  // SD/coreproject.f90:969-972
+  SDV_REQUIRE(low >= 1 && high <= N, SDV_ERR_ARG, "programmer fail: lowspf/highspf out of range");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t tr_bytes = (size_t)n3 * N * nnn * sizeof(z64);
+  const size_t tm_bytes = (size_t)N * N * N * nnn * sizeof(z64);
+  if (c->cfg.notwoflag) {  // :1019-1023
+    SDV_HIP_CHECK(hipMemsetAsync(twoereduced, 0, tr_bytes, st));
+    SDV_HIP_CHECK(hipMemsetAsync(twoematel, 0, tm_bytes, st));
+    return SDV_OK;
+  }
+  const bool sym = (phi1 == phi2);
+  const bool herm = c->cfg.cnormflag == 0;
+  // ---- pair bookkeeping: twoereduced(:, 2a, 2b) column (2a-1) + N (2b-low)
+  auto col = [&](int a2, int b2) { return (a2 - 1) + N * (b2 - low); };
+  std::vector<PairItem> items;
+  std::vector<int2> cmap((size_t)N * nnn, int2{-1, 0});
+  for (int b2 = low; b2 <= high; ++b2)
+    for (int a2 = 1; a2 <= N; ++a2) {
+      const bool mirror_needed = sym && a2 != b2 && a2 >= low && a2 <= high;
+      if (mirror_needed && a2 > b2) continue;  // produced as the mirror of (b2, a2)
+      const int ci = (int)items.size();
+      items.push_back(PairItem{a2 - 1, b2 - 1, col(a2, b2), mirror_needed ? col(b2, a2) : -1});
+      cmap[col(a2, b2)] = int2{ci, 0};
+      if (mirror_needed) cmap[col(b2, a2)] = int2{ci, herm ? 1 : 0};
+    }
+  SDV_TRY(run_pairs(c, items, phi1, phi2, 0, twoereduced, st));
+  // ---- rows of the reduction
+  std::vector<int2> rows;
+  std::vector<int2> rmap((size_t)N * N);
+  if (sym) {
+    std::vector<int> ridx((size_t)N * N, -1);
+    for (int b1 = 0; b1 < N; ++b1)
+      for (int a1 = 0; a1 <= b1; ++a1) {
+        ridx[a1 + N * b1] = (int)rows.size();
+        rows.push_back(int2{a1, b1});
+      }
+    for (int b1 = 0; b1 < N; ++b1)
+      for (int a1 = 0; a1 < N; ++a1)
+        rmap[a1 + N * b1] = a1 <= b1 ? int2{ridx[a1 + N * b1], 0} : int2{ridx[b1 + N * a1], herm ? 1 : 0};
+  } else {
+    for (int b1 = 0; b1 < N; ++b1)
+      for (int a1 = 0; a1 < N; ++a1) {
+        rmap[a1 + N * b1] = int2{(int)rows.size(), 0};
+        rows.push_back(int2{a1, b1});
+      }
+  }
+  std::vector<int> cols(items.size());
+  for (size_t i = 0; i < items.size(); ++i) cols[i] = items[i].col;
+  const int nrows = (int)rows.size(), ncols = (int)cols.size();
+  // index buffers live after the pair items in c->idx
+  const size_t off_items = items.size() * sizeof(PairItem);
+  const size_t off_rows = (off_items + 255) & ~(size_t)255;
+  const size_t off_cols = (off_rows + rows.size() * sizeof(int2) + 255) & ~(size_t)255;
+  const size_t off_rmap = (off_cols + cols.size() * sizeof(int) + 255) & ~(size_t)255;
+  const size_t off_cmap = (off_rmap + rmap.size() * sizeof(int2) + 255) & ~(size_t)255;
+  const size_t idx_bytes = off_cmap + cmap.size() * sizeof(int2);
+  // run_pairs already used c->idx for the items; grow keeps contents only if
+  // no realloc happens, so stage the matel indices in a separate region.
+  SDV_HIP_CHECK(hipStreamSynchronize(st));
+  SDV_TRY(ensure(c->idx, idx_bytes));
+  char* ib = (char*)c->idx.p;
+  SDV_HIP_CHECK(hipMemcpyAsync(ib + off_rows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+  SDV_HIP_CHECK(hipMemcpyAsync(ib + off_cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  SDV_HIP_CHECK(hipMemcpyAsync(ib + off_rmap, rmap.data(), rmap.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+  SDV_HIP_CHECK(hipMemcpyAsync(ib + off_cmap, cmap.data(), cmap.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+  SDV_TRY(ensure(c->scratch, matel_scratch_bytes(n3, nrows, ncols)));
+  SDV_TRY(ensure(c->gout, (size_t)(2 * nrows) * (2 * ncols) * sizeof(double)));
+  MatelArgsHost m;
+  m.n3 = n3;
+  m.N = N;
+  m.nrows = nrows;
+  m.rows_dev = (const int2*)(ib + off_rows);
+  m.ncols = ncols;
+  m.cols_dev = (const int*)(ib + off_cols);
+  m.phi1 = phi1;
+  m.phi2 = phi2;
+  m.conj_b = herm ? 1 : 0;
+  m.tr = twoereduced;
+  m.gout = (double*)c->gout.p;
+  m.scratch = c->scratch.p;
+  SDV_TRY(matel_launch(m, st));
+  const long tot = (long)N * N * N * nnn;
+  hipLaunchKernelGGL(k_assemble, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, N, nnn,
+                     (const int2*)(ib + off_rmap), (const int2*)(ib + off_cmap), (const double*)c->gout.p,
+                     2 * ncols, (z64*)twoematel);
+  SDV_HIP_CHECK(hipGetLastError());
+  // the index buffer is reused by the next call: finish before returning
+  SDV_HIP_CHECK(hipStreamSynchronize(st));
+  return SDV_OK;
+}
+
+int sdv_mult_ke_dev(sdv_ctx* c, const void* in, void* out, int howmany, void* stream) {
+  SDV_REQUIRE(c && in && out, SDV_ERR_ARG, "null argument");
+  SDV_REQUIRE(howmany >= 0, SDV_ERR_ARG, "howmany must be >= 0");
+  KeArgsHost a;
+  a.n = c->n;
+  a.howmany = howmany;
+  a.ke = c->ke;
+  a.invj = c->invj;
+  a.sdiag = c->sdiag;
+  a.in = in;
+  a.out = out;
+  return ke_launch(a, c->precision, (hipStream_t)stream);
+}
+
+int sdv_mult_reducedpot_dev(sdv_ctx* c, int first, int last, const void* inspfs, void* outspfs,
+                            const void* reducedpot, void* stream) {
+  SDV_REQUIRE(c && inspfs && outspfs && reducedpot, SDV_ERR_ARG, "null argument");
+  if (last < first) return SDV_OK;
+  return mult_reducedpot_launch(c->n3, c->N, last - first + 1, inspfs, reducedpot, outspfs, (hipStream_t)stream);
+}
+
+int sdv_get_reducedpot_dev(sdv_ctx* c, const void* tr, const void* twoden, void* rp, void* stream) {
+  SDV_REQUIRE(c && tr && twoden && rp, SDV_ERR_ARG, "null argument");
+  return reducedpot_launch(c->n3, c->N, tr, twoden, rp, (hipStream_t)stream);
+}
+
+// ---- host-pointer (Fortran drop-in) variants: stage, run, copy back, sync.
+static int stage_in(sdv_ctx* c, sdv_ctx::Buf& b, const void* h, size_t bytes, const void** d) {
+  SDV_TRY(ensure(b, bytes));
+  SDV_HIP_CHECK(hipMemcpyAsync(b.p, h, bytes, hipMemcpyDefault, c->stream));
+  *d = b.p;
+  return SDV_OK;
+}
+
+int sdv_op_tinv(sdv_ctx* c, const double* den, double* red, int allsize, int circsize) {
+  SDV_REQUIRE(c && den && red, SDV_ERR_ARG, "null argument");
+  const size_t bytes = (size_t)c->n3 * (allsize > 0 ? allsize : 0) * sizeof(z64);
+  const void* dd;
+  SDV_TRY(stage_in(c, c->h_in1, den, bytes, &dd));
+  SDV_TRY(ensure(c->h_out1, bytes));
+  SDV_TRY(sdv_op_tinv_dev(c, dd, c->h_out1.p, allsize, circsize, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(red, c->h_out1.p, bytes, hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+int sdv_call_twoe_matel00(sdv_ctx* c, int low, int high, const double* phi1, const double* phi2, double* tm,
+                          double* tr) {
+  SDV_REQUIRE(c && phi1 && phi2 && tm && tr, SDV_ERR_ARG, "null argument");
+  const int N = c->N, nnn = high - low + 1;
+  if (nnn <= 0) return SDV_OK;
+  const size_t pb = (size_t)c->n3 * N * sizeof(z64);
+  const void *d1, *d2;
+  SDV_TRY(stage_in(c, c->h_in1, phi1, pb, &d1));
+  if (phi2 == phi1)
+    d2 = d1;
+  else
+    SDV_TRY(stage_in(c, c->h_in2, phi2, pb, &d2));
+  const size_t trb = (size_t)c->n3 * N * nnn * sizeof(z64), tmb = (size_t)N * N * N * nnn * sizeof(z64);
+  SDV_TRY(ensure(c->h_out1, trb));
+  SDV_TRY(ensure(c->h_out2, tmb));
+  SDV_TRY(sdv_call_twoe_matel00_dev(c, low, high, d1, d2, c->h_out2.p, c->h_out1.p, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(tr, c->h_out1.p, trb, hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(tm, c->h_out2.p, tmb, hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+int sdv_mult_ke(sdv_ctx* c, const double* in, double* out, int howmany) {
+  SDV_REQUIRE(c && in && out, SDV_ERR_ARG, "null argument");
+  const size_t bytes = (size_t)c->n3 * (howmany > 0 ? howmany : 0) * sizeof(z64);
+  const void* di;
+  SDV_TRY(stage_in(c, c->h_in1, in, bytes, &di));
+  SDV_TRY(ensure(c->h_out1, bytes));
+  SDV_TRY(sdv_mult_ke_dev(c, di, c->h_out1.p, howmany, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(out, c->h_out1.p, bytes, hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+int sdv_mult_reducedpot(sdv_ctx* c, int first, int last, const double* inspfs, double* outspfs,
+                        const double* reducedpot) {
+  SDV_REQUIRE(c && inspfs && outspfs && reducedpot, SDV_ERR_ARG, "null argument");
+  if (last < first) return SDV_OK;
+  const int nout = last - first + 1;
+  const void *di, *dr;
+  SDV_TRY(stage_in(c, c->h_in1, inspfs, (size_t)c->n3 * c->N * sizeof(z64), &di));
+  SDV_TRY(stage_in(c, c->h_in2, reducedpot, (size_t)c->n3 * c->N * nout * sizeof(z64), &dr));
+  SDV_TRY(ensure(c->h_out1, (size_t)c->n3 * nout * sizeof(z64)));
+  SDV_TRY(sdv_mult_reducedpot_dev(c, first, last, di, c->h_out1.p, dr, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(outspfs, c->h_out1.p, (size_t)c->n3 * nout * sizeof(z64), hipMemcpyDefault,
+                               c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+int sdv_get_reducedpot(sdv_ctx* c, const double* tr, const double* twoden, double* rp) {
+  SDV_REQUIRE(c && tr && twoden && rp, SDV_ERR_ARG, "null argument");
+  const size_t NN = (size_t)c->N * c->N;
+  const void *dt, *dg;
+  SDV_TRY(stage_in(c, c->h_in1, tr, (size_t)c->n3 * NN * sizeof(z64), &dt));
+  SDV_TRY(stage_in(c, c->h_in2, twoden, NN * NN * sizeof(z64), &dg));
+  SDV_TRY(ensure(c->h_out1, (size_t)c->n3 * NN * sizeof(z64)));
+  SDV_TRY(sdv_get_reducedpot_dev(c, dt, dg, c->h_out1.p, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(rp, c->h_out1.p, (size_t)c->n3 * NN * sizeof(z64), hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,498 @@
+// sdv_conv.hip -- circulant-embedded Toeplitz 1/r convolution of orbital-pair
+// densities on gfx950 (the op_tinv / circ3d_sub path of the reference,
+// SD/coreproject.f90:366-537 and SD/toeplitz_subs.f90:561-589).
+//
+// The reference embeds each n^3 density into a (2n)^3 circulant and runs
+// 3 x (1-D FFT + transpose) forward, a pointwise multiply by the kernel
+// spectrum, and 3 x (1-D FFT + transpose) back (SD/toeplitz_subs.f90:264-294).
+// Here the circulant edge is M = 2m with m = P*Q >= n (any M >= 2n-1 gives
+// the same linear convolution), and the transform is pruned and fused:
+//
+//   k_xfwd : per pencil of x-lines: rho = phi_a * CONJ(phi_b) formed in
+//            registers (never stored), zero-padded FFT along x split into the
+//            even/odd frequency channels (two m-point FFTs), written as
+//            planes X[pair][kx][z][y].                      (R n^3, W 2m n^2)
+//   k_yz   : one (pair, kx) plane per workgroup, resident in LDS: y-forward,
+//            z-forward, multiply by the real even spectrum, z-inverse,
+//            y-inverse -- all pruned to the n x n data region.  (R+W 2m n^2)
+//   k_xinv : x-inverse of both channels, keeps x < n, writes V and its
+//            mirror column (conj for hermnorm) into twoereduced. (R 2m n^2, W n^3)
+//
+// Each m-point line FFT is a two-step Cooley-Tukey (P-point DFT in registers,
+// twiddle, LDS exchange, Q-point DFT in registers); frequency k1 + P k2 is
+// kept at position k1 Q + k2 and the inverse consumes that order directly.
+#include "sdv_dft.h"
+#include "sdv_internal.h"
+
+namespace sdv {
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cvt(z64 v) {
+  return mk((T)v.re, (T)v.im);
+}
+
+template <typename T>
+struct KArgs {
+  int n, n2, n3, npairs;
+  const z64* phi1;
+  const z64* phi2;
+  int density_mode, conj_b, mirror_conj;
+  const PairItem* pairs;
+  cplx<T>* X;
+  z64* out;
+  const cplx<T>* tw;
+  const T* khat;
+  int lds_tw2, lds_y;  // runtime LDS offsets (elements) of the 2-D twiddle table and workspace
+};
+
+template <int P, int Q>
+struct XCfg {
+  static constexpr int MX = P > Q ? P : Q;
+  static constexpr int LA = 256 / MX;  // x-lines per workgroup
+};
+
+template <int P, int Q>
+struct YZCfg {
+  static constexpr int R = P * Q;
+  static constexpr int MX = P > Q ? P : Q;
+  static constexpr int NT = ((R * MX + 63) / 64) * 64;
+};
+
+// ---------------------------------------------------------------- x forward
+template <typename T, int P, int Q>
+__device__ __forceinline__ void xfwd_step2(const KArgs<T>& a, const cplx<T>* buf, int pidx, int l0, int nl, int c) {
+  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1;
+  const int tid = threadIdx.x;
+  const int line = tid % LA, k1 = tid / LA;
+  if (k1 < P && line < nl) {
+    cplx<T> u[Q];
+#pragma unroll
+    for (int j2 = 0; j2 < Q; ++j2) u[j2] = buf[line * pitch + k1 * Q + j2];
+    DFT<T, Q, false>::run(u);
+    cplx<T>* dst = a.X + ((size_t)pidx * 2 * R + (size_t)c * R + (size_t)k1 * Q) * a.n2 + l0 + line;
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) dst[(size_t)k2 * a.n2] = u[k2];
+  }
+}
+
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
+  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + M;
+  const int tid = threadIdx.x;
+  const int n = a.n;
+  const int pidx = blockIdx.y;
+  const int l0 = blockIdx.x * LA;
+  const int nl = min(LA, a.n2 - l0);
+  const PairItem it = a.pairs[pidx];
+  for (int i = tid; i < M; i += 256) tw[i] = a.tw[i];
+  const z64* pa = a.phi1 + (size_t)it.a * a.n3 + (size_t)l0 * n;
+  const z64* pb = a.phi2 + (size_t)it.b * a.n3 + (size_t)l0 * n;
+  for (int e = tid; e < nl * n; e += 256) {
+    z64 v;
+    if (a.density_mode) {
+      v = pa[e];
+    } else {  // SD/coreproject.f90:1057-1065: inspfs10(:,a) * CONJUGATE(inspfs20(:,b))
+      z64 u = pa[e], w = pb[e];
+      if (a.conj_b) w.im = -w.im;
+      v = u * w;
+    }
+    const int line = e / n, x = e - line * n;
+    buf[line * pitch + x] = cvt<T>(v);
+  }
+  __syncthreads();
+  const int line = tid % LA, j2 = tid / LA;
+  const bool act = (j2 < Q) && (line < nl);
+  cplx<T> v[P], b[P];
+  if (act) {
+#pragma unroll
+    for (int j1 = 0; j1 < P; ++j1) {
+      const int x = j1 * Q + j2;
+      cplx<T> u = (x < n) ? buf[line * pitch + x] : mk((T)0, (T)0);
+      v[j1] = u;
+      b[j1] = u * tw[x];  // odd channel: x * w_M^x
+    }
+    DFT<T, P, false>::run(v);
+    DFT<T, P, false>::run(b);
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+      const cplx<T> w = tw[(2 * j2 * k1) % M];
+      v[k1] = v[k1] * w;
+      b[k1] = b[k1] * w;
+      buf[line * pitch + k1 * Q + j2] = v[k1];
+    }
+  }
+  __syncthreads();
+  xfwd_step2<T, P, Q>(a, buf, pidx, l0, nl, 0);
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) buf[line * pitch + k1 * Q + j2] = b[k1];
+  }
+  __syncthreads();
+  xfwd_step2<T, P, Q>(a, buf, pidx, l0, nl, 1);
+}
+
+// ------------------------------------------------------------- y/z planes
+// LDS: tw[M] = w_M^k ; T2[a*P + b] = w_m^(a b) (a < Q, b < P) ; Y = m x pitch
+// workspace.  The offsets of T2 and Y are runtime values, so the compiler
+// cannot prove the twiddle tables distinct from the workspace and reloads
+// them after each workspace store instead of pinning them in VGPRs.
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
+  constexpr int R = P * Q, NT = YZCfg<P, Q>::NT, pitch = R | 1, M = 2 * R;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* T2 = tw + a.lds_tw2;
+  cplx<T>* Y = tw + a.lds_y;
+  const int tid = threadIdx.x;
+  const int n = a.n, n2 = a.n2;
+  const int plane = blockIdx.x;
+  const int pidx = blockIdx.y;
+  cplx<T>* base = a.X + ((size_t)pidx * 2 * R + plane) * n2;
+  const int cx = plane / R, posx = plane - cx * R;
+  const int kx = 2 * (posx / Q + P * (posx % Q)) + cx;
+  const int kkx = min(kx, M - kx);
+  const T* kh = a.khat + (size_t)kkx * (R + 1) * (R + 1);
+  for (int i = tid; i < M; i += NT) tw[i] = a.tw[i];
+  for (int i = tid; i < R; i += NT) {
+    const int aa = i / P, bb = i - aa * P;
+    T2[i] = a.tw[(2 * aa * bb) % M];
+  }
+
+  const int yz0 = tid % n, yj0 = tid / n;  // y-line tasks: (z, j2) / (z, k1)
+  const int zr0 = tid % R, zj0 = tid / R;  // z-line tasks: (row, j2) / (row, k1)
+
+  // acc holds one ky channel's y-inverse output.  The cy = 0 result is parked
+  // in the plane's own global slot while cy = 1 runs (after the input has been
+  // staged), so no accumulator is live across the z pass.
+  cplx<T> acc[P];
+#pragma unroll 1
+  for (int cy = 0; cy < 2; ++cy) {
+    // Opaque copies of the task coordinates: index arithmetic derived from
+    // them is recomputed in each iteration instead of being hoisted and held
+    // in VGPRs across the z pass.
+    int yz = yz0, yj = yj0, zr = zr0, zj = zj0;
+    asm volatile("" : "+v"(yz), "+v"(yj), "+v"(zr), "+v"(zj));
+    const bool ya = yj < Q, yb = yj < P, za = zj < Q, zb = zj < P;
+    // stage the plane, layout [z][y]
+    __syncthreads();
+    for (int e = tid; e < n2; e += NT) {
+      const int z = e / n, y = e - z * n;
+      Y[z * pitch + y] = base[e];
+    }
+    __syncthreads();
+    cplx<T> v[P];
+    if (ya) {
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int y = j1 * Q + yj;
+        cplx<T> u = (y < n) ? Y[yz * pitch + y] : mk((T)0, (T)0);
+        v[j1] = cy ? u * tw[y] : u;
+      }
+    }
+    if (cy == 1) {  // park the cy = 0 output in global memory
+      __syncthreads();
+      if (ya) {
+#pragma unroll
+        for (int j1 = 0; j1 < P; ++j1) {
+          const int y = j1 * Q + yj;
+          if (y < n) Y[yz * pitch + y] = acc[j1];
+        }
+      }
+      __syncthreads();
+      for (int e = tid; e < n2; e += NT) {
+        const int z = e / n, y = e - z * n;
+        base[e] = Y[z * pitch + y];
+      }
+    }
+    __syncthreads();
+    // y forward, step 1 -> layout [y-position][z]
+    if (ya) {
+      DFT<T, P, false>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) Y[(k1 * Q + yj) * pitch + yz] = v[k1] * T2[yj * P + k1];
+    }
+    __syncthreads();
+    if (yb) {  // y forward, step 2
+      cplx<T> u[Q];
+#pragma unroll
+      for (int j2 = 0; j2 < Q; ++j2) u[j2] = Y[(yj * Q + j2) * pitch + yz];
+      DFT<T, Q, false>::run(u);
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) Y[(yj * Q + k2) * pitch + yz] = u[k2];
+    }
+    __syncthreads();
+    // z pass on row zr (frequency ky), both z channels.  Step 1 of both
+    // channels runs up front; channel 1's result waits in registers.
+    const int k1y = zr / Q, k2y = zr - (zr / Q) * Q;
+    const int ky = 2 * (k1y + P * k2y) + cy;
+    const int kky = min(ky, M - ky);
+    cplx<T> w1[P], zout[P];
+    if (za) {
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int z = j1 * Q + zj;
+        const cplx<T> u = (z < n) ? Y[zr * pitch + z] : mk((T)0, (T)0);
+        v[j1] = u;
+        w1[j1] = u * tw[z];
+      }
+      DFT<T, P, false>::run(v);
+      DFT<T, P, false>::run(w1);
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) {
+        const cplx<T> t = T2[zj * P + k1];
+        Y[zr * pitch + k1 * Q + zj] = v[k1] * t;
+        w1[k1] = w1[k1] * t;
+      }
+    }
+#pragma unroll
+    for (int cz = 0; cz < 2; ++cz) {
+      if (cz == 1 && za) {
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) Y[zr * pitch + k1 * Q + zj] = w1[k1];
+      }
+      __syncthreads();
+      if (zb) {
+        cplx<T> u[Q];
+#pragma unroll
+        for (int j2 = 0; j2 < Q; ++j2) u[j2] = Y[zr * pitch + zj * Q + j2];
+        DFT<T, Q, false>::run(u);
+#pragma unroll
+        for (int k2 = 0; k2 < Q; ++k2) {
+          const int kz = 2 * (zj + P * k2) + cz;
+          const int kkz = min(kz, M - kz);
+          u[k2] = scale(u[k2], kh[kky + (R + 1) * kkz]);
+        }
+        DFT<T, Q, true>::run(u);
+#pragma unroll
+        for (int j2 = 0; j2 < Q; ++j2) Y[zr * pitch + zj * Q + j2] = u[j2] * conj(T2[j2 * P + zj]);
+      }
+      __syncthreads();
+      if (za) {
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) v[k1] = Y[zr * pitch + k1 * Q + zj];
+        DFT<T, P, true>::run(v);
+        if (cz == 0) {
+#pragma unroll
+          for (int j1 = 0; j1 < P; ++j1) zout[j1] = v[j1];
+        } else {
+#pragma unroll
+          for (int j1 = 0; j1 < P; ++j1) zout[j1] = zout[j1] + v[j1] * conj(tw[j1 * Q + zj]);
+        }
+      }
+      // the next channel's writes touch exactly the positions this task read
+    }
+    if (za) {
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int z = j1 * Q + zj;
+        if (z < n) Y[zr * pitch + z] = zout[j1];
+      }
+    }
+    __syncthreads();
+    if (yb) {  // y inverse, step 1
+      cplx<T> u[Q];
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) u[k2] = Y[(yj * Q + k2) * pitch + yz];
+      DFT<T, Q, true>::run(u);
+#pragma unroll
+      for (int j2 = 0; j2 < Q; ++j2) Y[(yj * Q + j2) * pitch + yz] = u[j2] * conj(T2[j2 * P + yj]);
+    }
+    __syncthreads();
+    if (ya) {  // y inverse, step 2: this ky channel's output
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) v[k1] = Y[(k1 * Q + yj) * pitch + yz];
+      DFT<T, P, true>::run(v);
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) acc[j1] = cy ? v[j1] * conj(tw[j1 * Q + yj]) : v[j1];
+    }
+  }
+  // output = parked cy = 0 part + cy = 1 part
+  const int yz = yz0, yj = yj0;
+  const bool ya = yj < Q;
+  __syncthreads();
+  for (int e = tid; e < n2; e += NT) {
+    const int z = e / n, y = e - z * n;
+    Y[z * pitch + y] = base[e];
+  }
+  __syncthreads();
+  if (ya) {
+#pragma unroll
+    for (int j1 = 0; j1 < P; ++j1) {
+      const int y = j1 * Q + yj;
+      if (y < n) Y[yz * pitch + y] = Y[yz * pitch + y] + acc[j1];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < n2; e += NT) {
+    const int z = e / n, y = e - z * n;
+    base[e] = Y[z * pitch + y];
+  }
+}
+
+// ---------------------------------------------------------------- x inverse
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
+  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + M;
+  const int tid = threadIdx.x;
+  const int n = a.n;
+  const int pidx = blockIdx.y;
+  const int l0 = blockIdx.x * LA;
+  const int nl = min(LA, a.n2 - l0);
+  const PairItem it = a.pairs[pidx];
+  for (int i = tid; i < M; i += 256) tw[i] = a.tw[i];
+  const int line = tid % LA, jj = tid / LA;
+  cplx<T> acc[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) acc[j] = mk((T)0, (T)0);
+  for (int c = 0; c < 2; ++c) {
+    const cplx<T>* src = a.X + ((size_t)pidx * 2 * R + (size_t)c * R) * a.n2 + l0;
+    for (int e = tid; e < R * nl; e += 256) {
+      const int pos = e / nl, ln = e - pos * nl;
+      buf[ln * pitch + pos] = src[(size_t)pos * a.n2 + ln];
+    }
+    __syncthreads();
+    if (jj < P && line < nl) {  // inverse step 1: task (line, k1)
+      cplx<T> u[Q];
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) u[k2] = buf[line * pitch + jj * Q + k2];
+      DFT<T, Q, true>::run(u);
+#pragma unroll
+      for (int j2 = 0; j2 < Q; ++j2) buf[line * pitch + jj * Q + j2] = u[j2] * conj(tw[(2 * j2 * jj) % M]);
+    }
+    __syncthreads();
+    if (jj < Q && line < nl) {  // inverse step 2: task (line, j2)
+      cplx<T> v[P];
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) v[k1] = buf[line * pitch + k1 * Q + jj];
+      DFT<T, P, true>::run(v);
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) acc[j1] = acc[j1] + (c ? v[j1] * conj(tw[j1 * Q + jj]) : v[j1]);
+    }
+    __syncthreads();
+  }
+  if (jj < Q && line < nl) {
+#pragma unroll
+    for (int j1 = 0; j1 < P; ++j1) {
+      const int x = j1 * Q + jj;
+      if (x < n) buf[line * pitch + x] = acc[j1];
+    }
+  }
+  __syncthreads();
+  // twoereduced(:, col) and the mirror column (SD/coreproject.f90:521,528)
+  z64* o = a.out + (size_t)it.col * a.n3 + (size_t)l0 * n;
+  z64* om = it.mcol >= 0 ? a.out + (size_t)it.mcol * a.n3 + (size_t)l0 * n : nullptr;
+  for (int e = tid; e < nl * n; e += 256) {
+    const int ln = e / n, x = e - ln * n;
+    const cplx<T> v = buf[ln * pitch + x];
+    z64 d = mk((double)v.re, (double)v.im);
+    o[e] = d;
+    if (om) {
+      if (a.mirror_conj) d.im = -d.im;
+      om[e] = d;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- dispatch
+template <typename T, int P, int Q>
+static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st) {
+  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
+  KArgs<T> a;
+  a.n = h.n;
+  a.n2 = h.n * h.n;
+  a.n3 = h.n * h.n * h.n;
+  a.npairs = h.npairs;
+  a.phi1 = (const z64*)h.phi1;
+  a.phi2 = (const z64*)h.phi2;
+  a.density_mode = h.density_mode;
+  a.conj_b = h.conj_b;
+  a.mirror_conj = h.mirror_conj;
+  a.pairs = h.pairs_dev;
+  a.X = (cplx<T>*)h.work;
+  a.out = (z64*)h.out;
+  a.tw = (const cplx<T>*)(sizeof(T) == 8 ? s.tw64 : s.tw32);
+  a.khat = (const T*)(sizeof(T) == 8 ? (const void*)s.khat64 : (const void*)s.khat32);
+  const size_t shX = (size_t)(M + LA * pitch) * sizeof(cplx<T>);
+  a.lds_tw2 = M;
+  a.lds_y = M + R + ((M + R) & 1);  // keep the workspace 16-byte aligned for FP32
+  const size_t shB = (size_t)(a.lds_y + R * pitch) * sizeof(cplx<T>);
+  SDV_REQUIRE(shB <= 160 * 1024, SDV_ERR_UNSUPPORTED, "y/z plane does not fit in LDS");
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
+  dim3 gX((a.n2 + LA - 1) / LA, h.npairs);
+  dim3 gB(2 * R, h.npairs);
+  hipLaunchKernelGGL((k_xfwd<T, P, Q>), gX, dim3(256), shX, st, a);
+  SDV_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL((k_yz<T, P, Q>), gB, dim3(YZCfg<P, Q>::NT), shB, st, a);
+  SDV_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL((k_xinv<T, P, Q>), gX, dim3(256), shX, st, a);
+  SDV_HIP_CHECK(hipGetLastError());
+  return SDV_OK;
+}
+
+// Supported half-edges m = P*Q (smallest m >= n is chosen).
+struct PQ {
+  int m, P, Q;
+};
+static const PQ kSizes[] = {{9, 3, 3},  {12, 4, 3}, {16, 4, 4}, {21, 7, 3}, {25, 5, 5},
+                            {27, 9, 3}, {32, 8, 4}, {36, 6, 6}, {45, 9, 5}, {48, 8, 6},
+                            {54, 9, 6}, {63, 9, 7}, {64, 8, 8}, {72, 9, 8}, {81, 9, 9}};
+
+bool choose_geom(int n, Geom* g) {
+  for (const PQ& p : kSizes)
+    if (p.m >= n) {
+      g->n = n;
+      g->m = p.m;
+      g->M = 2 * p.m;
+      g->P = p.P;
+      g->Q = p.Q;
+      return true;
+    }
+  return false;
+}
+
+size_t conv_work_bytes(const Geom& g, int npairs, int precision) {
+  const size_t es = precision == 0 ? sizeof(z64) : sizeof(c32);
+  return (size_t)npairs * 2 * g.m * (size_t)g.n * g.n * es;
+}
+
+template <typename T>
+static int conv_dispatch(const Spectrum& s, const ConvArgsHost& h, hipStream_t st) {
+  switch (s.g.m) {
+#define SDV_CASE(m_, P_, Q_) \
+  case m_:                   \
+    return launch_pq<T, P_, Q_>(s, h, st);
+    SDV_CASE(9, 3, 3)
+    SDV_CASE(12, 4, 3)
+    SDV_CASE(16, 4, 4)
+    SDV_CASE(21, 7, 3)
+    SDV_CASE(25, 5, 5)
+    SDV_CASE(27, 9, 3)
+    SDV_CASE(32, 8, 4)
+    SDV_CASE(36, 6, 6)
+    SDV_CASE(45, 9, 5)
+    SDV_CASE(48, 8, 6)
+    SDV_CASE(54, 9, 6)
+    SDV_CASE(63, 9, 7)
+    SDV_CASE(64, 8, 8)
+    SDV_CASE(72, 9, 8)
+    SDV_CASE(81, 9, 9)
+#undef SDV_CASE
+    default:
+      set_error("unsupported circulant half-edge");
+      return SDV_ERR_UNSUPPORTED;
+  }
+}
+
+int conv_launch(const Spectrum& s, const ConvArgsHost& h, int precision, hipStream_t st) {
+  if (h.npairs <= 0) return SDV_OK;
+  return precision == 0 ? conv_dispatch<double>(s, h, st) : conv_dispatch<float>(s, h, st);
+}
+
+}  // namespace sdv

@@ -1,0 +1,90 @@
+// sdv_internal.h -- internal (C++) interfaces between the library's
+// translation units.  Nothing here crosses the C-ABI.
+#pragma once
+#include <vector>
+
+#include "sdv_common.h"
+
+namespace sdv {
+
+// One convolution work item: output column `col` of the (n^3, ncols) result
+// gets K * rho where rho = phi1[:,a] * CONJUGATE(phi2[:,b]) (orbital mode) or
+// rho = den[:,a] (density mode).  mcol >= 0 also receives the mirror pair
+// (b,a): conj(V) for hermnorm, V for cnorm.
+struct PairItem {
+  int a, b, col, mcol;
+};
+
+// Geometry of one circulant: grid edge n, half edge m = P*Q >= n, M = 2m.
+struct Geom {
+  int n = 0, m = 0, M = 0, P = 0, Q = 0;
+};
+bool choose_geom(int n, Geom* g);
+
+// Device-resident tables for one geometry.
+struct Spectrum {
+  Geom g;
+  void* tw64 = nullptr;     // M x z64   exp(-2 pi i k / M)
+  void* tw32 = nullptr;     // M x c32
+  double* khat64 = nullptr; // (m+1)^3, [kkx][kkz][kky], includes 1/M^3
+  float* khat32 = nullptr;
+  double* koct = nullptr;   // n^3 octant of the real-space kernel K(i,j,k), i,j,k >= 0
+};
+
+struct ConvArgsHost {
+  int n;
+  int npairs;
+  const void* phi1;   // z64 (n^3, N) or density (n^3, ncols)
+  const void* phi2;
+  int density_mode;   // 1: phi1 columns are densities
+  int conj_b;         // 1: conjugate the second factor (hermnorm)
+  int mirror_conj;    // 1: mirror columns receive conj(V)
+  const PairItem* pairs_dev;
+  void* work;         // intermediate, npairs * 2m * n^2 complex<T>
+  void* out;          // z64 (n^3, ncols)
+};
+
+// Convolution pipeline: x-forward (pair density fused), y/z plane kernel with
+// the spectral multiply, x-inverse with output scatter.  precision 0 = FP64,
+// 1 = FP32 butterflies.
+int conv_launch(const Spectrum& s, const ConvArgsHost& a, int precision, hipStream_t st);
+size_t conv_work_bytes(const Geom& g, int npairs, int precision);
+
+// Kernel construction (tooth + placement + spectrum), FP64.
+int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st);
+void free_spectrum(Spectrum* s);
+
+// twoematel reduction: G = sum_x D_row(x) V_col(x)
+struct MatelArgsHost {
+  int n3, N;
+  int nrows;                 // number of row pairs
+  const int2* rows_dev;      // (1a, 1b) 0-based, rho = phi1[1a] * CONJ(phi2[1b])
+  int ncols;
+  const int* cols_dev;       // twoereduced column index of each computed V
+  const void* phi1;
+  const void* phi2;
+  int conj_b;
+  const void* tr;            // z64 twoereduced base
+  double* gout;              // (2*nrows) x (2*ncols) real products, row-major
+  void* scratch;             // matel_scratch_bytes()
+};
+size_t matel_scratch_bytes(int n3, int nrows, int ncols);
+int matel_launch(const MatelArgsHost& a, hipStream_t st);
+
+// Kinetic apply
+struct KeArgsHost {
+  int n, howmany;
+  const double* ke;          // 2n-1 stencil (device)
+  const void* invj;          // 3 x n z64 per-axis 1/J (device) or null
+  const void* sdiag;         // 3 x n z64 per-axis scalediag terms (device) or null
+  const void* in;            // z64 (n^3, howmany)
+  void* out;
+};
+int ke_launch(const KeArgsHost& a, int precision, hipStream_t st);
+
+// Mean field
+int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, hipStream_t st);
+int mult_reducedpot_launch(int n3, int N, int nout, const void* inspfs, const void* rp, void* out,
+                           hipStream_t st);
+
+}  // namespace sdv

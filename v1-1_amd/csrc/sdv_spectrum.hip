@@ -1,0 +1,285 @@
+// sdv_spectrum.hip -- device construction of the Toeplitz 1/r Coulomb kernel
+// and of its circulant spectrum, in FP64.
+//
+// Restates GETINVERSE / RVECTOR / TOOTHEIGMULT (SD/tooth.f90:12-231):
+//   R(i,j,k)  = D^3 / |(iD, jD, kD)|,  R(0) = 0                    (:135-146)
+//   TR        = -sum_p [T(i-p) R(p,j,k) + T(j-p) R(i,p,k) + T(k-p) R(i,j,p)]
+//               on the small box |i|,|j|,|k| <= nsmall, p in [-nbig, nbig]
+//               (:158-166, with the intended slice -NBIG-J:NBIG-J at :165)
+//   TR(0)    += 2 pi                                                 (:44)
+//   T_small   = 1-D sinc Laplacian on 2 nsmall + 1 points            (:48-53)
+//   Q         = W^(x3) [ (W^T)^(x3) TR / (E_i + E_j + E_k) ]         (:63-79)
+//   TINV      = R + Q on the small box, R elsewhere                  (:83-89)
+// then threed_two = TINV / D^3 (SD/sincproject.f90:155-156).
+//
+// The eigenpairs of T_small come from a cyclic Jacobi sweep run in one
+// workgroup (the reference calls LAPACK DSYEV, :64); Q is the unique solution
+// of the linear system, so any accurate eigen-decomposition gives it.
+//
+// The circulant holds K at offsets |o| <= n-1 on each axis (real, even), so
+// its DFT is real and even:  Khat(k) = sum_{i,j,l} w_i w_j w_l K(i,j,l)
+// cos(2pi k_x i/M) cos(2pi k_y j/M) cos(2pi k_z l/M) with w_0 = 1, w_i = 2.
+// It is evaluated as three separable cosine contractions on the octant and
+// stored for k in [0, m]^3 with the 1/M^3 of SD/toeplitz_subs.f90:578.
+#include "sdv_internal.h"
+
+namespace sdv {
+
+__device__ __forceinline__ double rvec(int i, int j, int k, double delta) {
+  if (i == 0 && j == 0 && k == 0) return 0.0;
+  const double pi_ = i * delta, pj = j * delta, pk = k * delta;
+  return (delta * delta * delta) / sqrt(pi_ * pi_ + pj * pj + pk * pk);
+}
+
+__device__ __forceinline__ double alltt(int m, double delta) {
+  if (m == 0) return (kPiD * kPiD) / (6.0 * delta * delta);
+  const double sgn = (m & 1) ? -1.0 : 1.0;
+  return sgn * (1.0 / (delta * delta)) / ((double)m * (double)m);
+}
+
+__global__ void k_tr(double* TR, int nsmall, int nbig, double delta) {
+  const int NS = 2 * nsmall + 1;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)NS * NS * NS) return;
+  const int i = (int)(idx % NS) - nsmall;
+  const int j = (int)((idx / NS) % NS) - nsmall;
+  const int k = (int)(idx / ((long)NS * NS)) - nsmall;
+  double s = 0.0;
+  for (int p = -nbig; p <= nbig; ++p)
+    s += alltt(p - i, delta) * rvec(p, j, k, delta) + alltt(p - j, delta) * rvec(i, p, k, delta) +
+         alltt(p - k, delta) * rvec(i, j, p, delta);
+  double v = -s;
+  if (i == 0 && j == 0 && k == 0) v += 2.0 * kPiD;
+  TR[idx] = v;
+}
+
+// One-workgroup cyclic Jacobi on the NS x NS sinc Laplacian (column-major).
+// Outputs eigenvectors W (columns) and eigenvalues E.
+__global__ void __launch_bounds__(256) k_jacobi(int nsmall, double delta, double* Wout, double* Eout) {
+  const int NS = 2 * nsmall + 1;
+  extern __shared__ double sh[];
+  double* A = sh;             // NS*NS
+  double* W = sh + NS * NS;   // NS*NS
+  double* red = W + NS * NS;  // 256
+  __shared__ double cs[2];
+  const int tid = threadIdx.x;
+  const double mycon = (kPiD / delta) * (kPiD / delta) / 6.0;
+  for (int e = tid; e < NS * NS; e += 256) {
+    const int i = e % NS, j = e / NS;
+    double t;
+    if (i == j)
+      t = mycon;
+    else {
+      const int d = i - j;
+      t = ((d & 1) ? -1.0 : 1.0) / (delta * delta * (double)d * (double)d);
+    }
+    A[e] = t;
+    W[e] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int e = tid; e < NS * NS; e += 256) {
+      const double v = A[e] * A[e];
+      tot += v;
+      if (e % NS != e / NS) off += v;
+    }
+    red[tid] = off;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    const double offs = red[0];
+    __syncthreads();
+    red[tid] = tot;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    const double tots = red[0];
+    __syncthreads();
+    if (offs <= 1e-60 * tots) break;
+    for (int p = 0; p < NS - 1; ++p)
+      for (int q = p + 1; q < NS; ++q) {
+        if (tid == 0) {
+          const double apq = A[p + NS * q];
+          double c = 1.0, s = 0.0;
+          if (fabs(apq) >= 1e-300) {
+            const double tau = (A[q + NS * q] - A[p + NS * p]) / (2.0 * apq);
+            const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+          }
+          cs[0] = c;
+          cs[1] = s;
+        }
+        __syncthreads();
+        const double c = cs[0], s = cs[1];
+        if (s != 0.0) {
+          for (int k = tid; k < NS; k += 256) {  // columns p, q of A and W
+            const double akp = A[k + NS * p], akq = A[k + NS * q];
+            A[k + NS * p] = c * akp - s * akq;
+            A[k + NS * q] = s * akp + c * akq;
+            const double wkp = W[k + NS * p], wkq = W[k + NS * q];
+            W[k + NS * p] = c * wkp - s * wkq;
+            W[k + NS * q] = s * wkp + c * wkq;
+          }
+          __syncthreads();
+          for (int k = tid; k < NS; k += 256) {  // rows p, q of A
+            const double apk = A[p + NS * k], aqk = A[q + NS * k];
+            A[p + NS * k] = c * apk - s * aqk;
+            A[q + NS * k] = s * apk + c * aqk;
+          }
+        }
+        __syncthreads();
+      }
+  }
+  for (int e = tid; e < NS * NS; e += 256) Wout[e] = W[e];
+  for (int i = tid; i < NS; i += 256) Eout[i] = A[i + NS * i];
+}
+
+// Y(.., a, ..) = sum_l Wm(a, l) X(.., l, ..) along one axis of an NS^3 cube;
+// trans = 1 uses Wm^T.
+__global__ void k_axis_mat(const double* X, double* Y, const double* Wm, int NS, int axis, int trans) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n3 = (long)NS * NS * NS;
+  if (idx >= n3) return;
+  int c[3] = {(int)(idx % NS), (int)((idx / NS) % NS), (int)(idx / ((long)NS * NS))};
+  const long stride = axis == 0 ? 1 : (axis == 1 ? NS : (long)NS * NS);
+  const int a = c[axis];
+  const long base = idx - (long)a * stride;
+  double s = 0.0;
+  for (int l = 0; l < NS; ++l) {
+    const double w = trans ? Wm[l + NS * a] : Wm[a + NS * l];
+    s += w * X[base + (long)l * stride];
+  }
+  Y[idx] = s;
+}
+
+__global__ void k_eig_divide(double* X, const double* E, int NS) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)NS * NS * NS) return;
+  const int i = idx % NS, j = (idx / NS) % NS, k = idx / ((long)NS * NS);
+  X[idx] = X[idx] / (E[i] + E[j] + E[k]);
+}
+
+// K on the first octant (i,j,k in [0,n)), divided by D^3.
+__global__ void k_kernel_octant(double* koct, const double* Q, int n, int nsmall, double delta) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n * n) return;
+  const int i = idx % n, j = (idx / n) % n, k = idx / ((long)n * n);
+  double v = rvec(i, j, k, delta);
+  if (i <= nsmall && j <= nsmall && k <= nsmall) {
+    const int NS = 2 * nsmall + 1;
+    v += Q[(i + nsmall) + (long)NS * ((j + nsmall) + (long)NS * (k + nsmall))];
+  }
+  koct[idx] = v / (delta * delta * delta);
+}
+
+// out(a, k, b) = sum_i C(k, i) in(a, i, b); in dims (A, I, B), out (A, K, B)
+__global__ void k_cos_axis(const double* in, double* out, int A, int I, int B, int K, int M) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)A * K * B) return;
+  const int a = idx % A;
+  const int k = (idx / A) % K;
+  const long b = idx / ((long)A * K);
+  double s = 0.0;
+  for (int i = 0; i < I; ++i) {
+    const int r = (int)(((long)k * i) % M);
+    const double c = cospi(2.0 * (double)r / (double)M) * (i == 0 ? 1.0 : 2.0) / (double)M;
+    s += c * in[a + (long)A * (i + (long)I * b)];
+  }
+  out[idx] = s;
+}
+
+// S[kx + m1 (ky + m1 kz)] -> khat[kky + m1 (kkz + m1 kkx)], plus FP32 copy
+__global__ void k_khat_permute(const double* S, double* kh64, float* kh32, int m1) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)m1 * m1 * m1) return;
+  const int kx = idx % m1, ky = (idx / m1) % m1, kz = idx / ((long)m1 * m1);
+  const long o = ky + (long)m1 * (kz + (long)m1 * kx);
+  kh64[o] = S[idx];
+  kh32[o] = (float)S[idx];
+}
+
+__global__ void k_twiddles(z64* tw64, c32* tw32, int M) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  double s, c;
+  sincospi(2.0 * (double)k / (double)M, &s, &c);
+  tw64[k] = mk(c, -s);
+  tw32[k] = mk((float)c, (float)-s);
+}
+
+static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st) {
+  SDV_REQUIRE(nbig > nsmall, SDV_ERR_ARG, "toothnbig must exceed toothnsmall (SD/tooth.f90:114-117)");
+  SDV_REQUIRE(nbig >= n - 1, SDV_ERR_ARG, "toothnbig must be >= numpoints-1 (SD/sincproject.f90:151-153)");
+  SDV_REQUIRE(choose_geom(n, &s->g), SDV_ERR_UNSUPPORTED, "grid edge beyond the supported circulant sizes");
+  const Geom& g = s->g;
+  const int NS = 2 * nsmall + 1;
+  const long ns3 = (long)NS * NS * NS;
+  const long n3 = (long)n * n * n;
+  const int m1 = g.m + 1;
+  double *TR = nullptr, *T2 = nullptr, *W = nullptr, *E = nullptr, *S1 = nullptr, *S2 = nullptr, *S3 = nullptr;
+  SDV_HIP_CHECK(hipMalloc(&TR, ns3 * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&T2, ns3 * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&W, (size_t)NS * NS * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&E, (size_t)NS * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&s->koct, n3 * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&S1, (size_t)m1 * n * n * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&S2, (size_t)m1 * m1 * n * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&S3, (size_t)m1 * m1 * m1 * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&s->khat64, (size_t)m1 * m1 * m1 * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&s->khat32, (size_t)m1 * m1 * m1 * sizeof(float)));
+  SDV_HIP_CHECK(hipMalloc(&s->tw64, (size_t)g.M * sizeof(z64)));
+  SDV_HIP_CHECK(hipMalloc(&s->tw32, (size_t)g.M * sizeof(c32)));
+
+  hipLaunchKernelGGL(k_tr, dim3(nblk(ns3, 256)), dim3(256), 0, st, TR, nsmall, nbig, spacing);
+  const size_t shj = (2 * (size_t)NS * NS + 256) * sizeof(double);
+  SDV_REQUIRE(shj <= 160 * 1024, SDV_ERR_UNSUPPORTED, "toothnsmall too large for the LDS eigensolver");
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_jacobi, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shj));
+  hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(256), shj, st, nsmall, spacing, W, E);
+  // QQ = (W^T)^(x3) TR ; QQ /= E_i+E_j+E_k ; Q = W^(x3) QQ   (TOOTHEIGMULT twice)
+  hipLaunchKernelGGL(k_axis_mat, dim3(nblk(ns3, 256)), dim3(256), 0, st, TR, T2, W, NS, 2, 1);
+  hipLaunchKernelGGL(k_axis_mat, dim3(nblk(ns3, 256)), dim3(256), 0, st, T2, TR, W, NS, 1, 1);
+  hipLaunchKernelGGL(k_axis_mat, dim3(nblk(ns3, 256)), dim3(256), 0, st, TR, T2, W, NS, 0, 1);
+  hipLaunchKernelGGL(k_eig_divide, dim3(nblk(ns3, 256)), dim3(256), 0, st, T2, E, NS);
+  hipLaunchKernelGGL(k_axis_mat, dim3(nblk(ns3, 256)), dim3(256), 0, st, T2, TR, W, NS, 2, 0);
+  hipLaunchKernelGGL(k_axis_mat, dim3(nblk(ns3, 256)), dim3(256), 0, st, TR, T2, W, NS, 1, 0);
+  hipLaunchKernelGGL(k_axis_mat, dim3(nblk(ns3, 256)), dim3(256), 0, st, T2, TR, W, NS, 0, 0);
+  hipLaunchKernelGGL(k_kernel_octant, dim3(nblk(n3, 256)), dim3(256), 0, st, s->koct, TR, n, nsmall, spacing);
+  // separable cosine transform of the octant
+  hipLaunchKernelGGL(k_cos_axis, dim3(nblk((long)m1 * n * n, 256)), dim3(256), 0, st, s->koct, S1, 1, n, n * n, m1,
+                     g.M);
+  hipLaunchKernelGGL(k_cos_axis, dim3(nblk((long)m1 * m1 * n, 256)), dim3(256), 0, st, S1, S2, m1, n, n, m1, g.M);
+  hipLaunchKernelGGL(k_cos_axis, dim3(nblk((long)m1 * m1 * m1, 256)), dim3(256), 0, st, S2, S3, m1 * m1, n, 1, m1,
+                     g.M);
+  hipLaunchKernelGGL(k_khat_permute, dim3(nblk((long)m1 * m1 * m1, 256)), dim3(256), 0, st, S3, s->khat64,
+                     s->khat32, m1);
+  hipLaunchKernelGGL(k_twiddles, dim3(nblk(g.M, 256)), dim3(256), 0, st, (z64*)s->tw64, (c32*)s->tw32, g.M);
+  SDV_HIP_CHECK(hipGetLastError());
+  SDV_HIP_CHECK(hipStreamSynchronize(st));
+  hipFree(TR);
+  hipFree(T2);
+  hipFree(W);
+  hipFree(E);
+  hipFree(S1);
+  hipFree(S2);
+  hipFree(S3);
+  return SDV_OK;
+}
+
+void free_spectrum(Spectrum* s) {
+  hipFree(s->tw64);
+  hipFree(s->tw32);
+  hipFree(s->khat64);
+  hipFree(s->khat32);
+  hipFree(s->koct);
+  *s = Spectrum();
+}
+
+}  // namespace sdv

@@ -1,0 +1,196 @@
+"""Host-side mirror of the SINCDVR project interface (the reference's
+`mult_ke`, `op_tinv`, `call_twoe_matel00`, `mult_reducedpot` and the
+`get_reducedpot0` GEMM), over the C-ABI of libsincdvr.so.
+
+Array convention.  The reference passes column-major complex*16 arrays.  Here:
+  * numpy arrays are Fortran-ordered complex128 with the reference's shapes,
+    e.g. inspfs (totpoints, numspf); they go through the host-pointer entry
+    points (synchronous, like the Fortran call);
+  * torch tensors are complex128 on the GPU with the reversed (C-order) shape,
+    e.g. inspfs (numspf, totpoints), so the storage is identical to the
+    Fortran array; they go through the *_dev entry points on the current
+    torch stream.
+Errors raise SdvError carrying the library's message, where the reference
+would print and stop (CFLST).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is present in this image
+    torch = None
+
+
+def _is_torch(x):
+    return torch is not None and isinstance(x, torch.Tensor)
+
+
+def _dptr(t):
+    assert t.is_cuda and t.is_contiguous() and t.dtype == torch.complex128, (t.device, t.dtype)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _hptr(a):
+    assert a.dtype == np.complex128 and (a.flags["F_CONTIGUOUS"] or a.ndim <= 1), (a.dtype, a.flags)
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class SincDVR:
+    """One sinc-DVR grid with its device-resident Coulomb kernel spectrum, KE
+    stencil and ECS tables (the project state of myprojectmod / myparams)."""
+
+    def __init__(self, numpoints, spacing, numspf, cnorm=False, scaling=None, toothnbig=240,
+                 toothnsmall=40, notwoflag=0, debugflag=0, precision=0):
+        lib = _lib.load()
+        cfg = _lib.SdvConfig()
+        lib.sdv_config_default(ctypes.byref(cfg))
+        cfg.numpoints = numpoints
+        cfg.spacing = spacing
+        cfg.numspf = numspf
+        cfg.toothnbig = toothnbig
+        cfg.toothnsmall = toothnsmall
+        cfg.cnormflag = int(bool(cnorm))
+        cfg.notwoflag = notwoflag
+        cfg.debugflag = debugflag
+        cfg.precision = precision
+        if scaling is not None:
+            cfg.scalingflag = 1
+            cfg.scalingtheta = scaling["theta"]
+            cfg.scalingstretch = scaling.get("stretch", 1.0)
+            cfg.scalingdistance = scaling["dist"]
+            cfg.smoothness = scaling["smooth"]
+        h = ctypes.c_void_p()
+        _lib.check(lib.sdv_init(ctypes.byref(cfg), ctypes.byref(h)))
+        self._lib = lib
+        self._h = h
+        self.n = numpoints
+        self.totpoints = numpoints ** 3
+        self.numspf = numspf
+        self.cnorm = bool(cnorm)
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sdv_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, p):
+        _lib.check(self._lib.sdv_set_precision(self._h, int(p)))
+        self.precision = p
+
+    @property
+    def circulant_edge(self):
+        return self._lib.sdv_circulant_edge(self._h)
+
+    def kernel_octant(self):
+        """threed_two(i,j,k) for i,j,k in [0,n): (n,n,n) Fortran array."""
+        out = np.zeros((self.n,) * 3, order="F")
+        _lib.check(self._lib.sdv_get_kernel_octant(self._h, ctypes.c_void_p(out.ctypes.data)))
+        return out
+
+    def ke_stencil(self):
+        out = np.zeros(2 * self.n - 1)
+        _lib.check(self._lib.sdv_get_ke_stencil(self._h, ctypes.c_void_p(out.ctypes.data)))
+        return out
+
+    # ---------------------------------------------------------------- op_tinv
+    def op_tinv(self, twoeden03, circsize=None):
+        """SD/coreproject.f90:250-258.  numpy (totpoints, allsize) F-order or
+        torch (allsize, totpoints) -> twoereduced of the same layout."""
+        if _is_torch(twoeden03):
+            t = twoeden03.reshape(-1, self.totpoints)
+            allsize = t.shape[0]
+            out = torch.empty_like(t)
+            _lib.check(self._lib.sdv_op_tinv_dev(self._h, _dptr(t), _dptr(out), allsize,
+                                                 circsize or allsize, _stream()))
+            return out.reshape(twoeden03.shape)
+        a = np.asfortranarray(twoeden03, dtype=np.complex128).reshape(self.totpoints, -1, order="F")
+        allsize = a.shape[1]
+        out = np.zeros_like(a, order="F")
+        _lib.check(self._lib.sdv_op_tinv(self._h, _hptr(a), _hptr(out), allsize, circsize or allsize))
+        return out
+
+    # ------------------------------------------------------- call_twoe_matel00
+    def call_twoe_matel00(self, lowspf, highspf, inspfs10, inspfs20=None, out=None):
+        """SD/coreproject.f90:945-955.  Returns (twoematel, twoereduced).
+        numpy: twoematel (N,N,N,nnn) and twoereduced (totpoints,N,nnn), F-order.
+        torch: twoematel (nnn,N,N,N) and twoereduced (nnn,N,totpoints).
+        Passing the same array twice (or inspfs20=None) uses pair symmetry."""
+        N = self.numspf
+        nnn = highspf - lowspf + 1
+        same = inspfs20 is None or inspfs20 is inspfs10
+        if _is_torch(inspfs10):
+            p1 = inspfs10.contiguous()
+            p2 = p1 if same else inspfs20.contiguous()
+            if out is None:
+                tm = torch.zeros((max(nnn, 0), N, N, N), dtype=torch.complex128, device=p1.device)
+                tr = torch.zeros((max(nnn, 0), N, self.totpoints), dtype=torch.complex128, device=p1.device)
+            else:
+                tm, tr = out
+            _lib.check(self._lib.sdv_call_twoe_matel00_dev(self._h, lowspf, highspf, _dptr(p1), _dptr(p2),
+                                                           _dptr(tm), _dptr(tr), _stream()))
+            return tm, tr
+        p1 = np.asfortranarray(inspfs10, dtype=np.complex128)
+        p2 = p1 if same else np.asfortranarray(inspfs20, dtype=np.complex128)
+        tm = np.zeros((N, N, N, max(nnn, 0)), dtype=np.complex128, order="F")
+        tr = np.zeros((self.totpoints, N, max(nnn, 0)), dtype=np.complex128, order="F")
+        _lib.check(self._lib.sdv_call_twoe_matel00(self._h, lowspf, highspf, _hptr(p1), _hptr(p2), _hptr(tm),
+                                                   _hptr(tr)))
+        return tm, tr
+
+    # ---------------------------------------------------------------- mult_ke
+    def mult_ke(self, inp):
+        """SD/coreproject.f90:2104-2116."""
+        if _is_torch(inp):
+            t = inp.reshape(-1, self.totpoints).contiguous()
+            out = torch.empty_like(t)
+            _lib.check(self._lib.sdv_mult_ke_dev(self._h, _dptr(t), _dptr(out), t.shape[0], _stream()))
+            return out.reshape(inp.shape)
+        a = np.asfortranarray(inp, dtype=np.complex128).reshape(self.totpoints, -1, order="F")
+        out = np.zeros_like(a, order="F")
+        _lib.check(self._lib.sdv_mult_ke(self._h, _hptr(a), _hptr(out), a.shape[1]))
+        return out
+
+    # -------------------------------------------------------- mult_reducedpot
+    def mult_reducedpot(self, firstspf, lastspf, inspfs, reducedpot):
+        """SD/coreproject.f90:1200-1235."""
+        nout = lastspf - firstspf + 1
+        if _is_torch(inspfs):
+            out = torch.zeros((max(nout, 0), self.totpoints), dtype=torch.complex128, device=inspfs.device)
+            _lib.check(self._lib.sdv_mult_reducedpot_dev(self._h, firstspf, lastspf, _dptr(inspfs.contiguous()),
+                                                         _dptr(out), _dptr(reducedpot.contiguous()), _stream()))
+            return out
+        p = np.asfortranarray(inspfs, dtype=np.complex128)
+        rp = np.asfortranarray(reducedpot, dtype=np.complex128)
+        out = np.zeros((self.totpoints, max(nout, 0)), dtype=np.complex128, order="F")
+        _lib.check(self._lib.sdv_mult_reducedpot(self._h, firstspf, lastspf, _hptr(p), _hptr(out), _hptr(rp)))
+        return out
+
+    # -------------------------------------------------- get_reducedpot0 GEMM
+    def get_reducedpot(self, twoereduced, twoden):
+        """S/mean.f90:556-558: reducedpot = twoereduced . twoden."""
+        N = self.numspf
+        if _is_torch(twoereduced):
+            out = torch.empty((N, N, self.totpoints), dtype=torch.complex128, device=twoereduced.device)
+            _lib.check(self._lib.sdv_get_reducedpot_dev(self._h, _dptr(twoereduced.contiguous()),
+                                                        _dptr(twoden.contiguous()), _dptr(out), _stream()))
+            return out
+        tr = np.asfortranarray(twoereduced, dtype=np.complex128)
+        g = np.asfortranarray(twoden, dtype=np.complex128)
+        out = np.zeros((self.totpoints, N, N), dtype=np.complex128, order="F")
+        _lib.check(self._lib.sdv_get_reducedpot(self._h, _hptr(tr), _hptr(g), _hptr(out)))
+        return out
