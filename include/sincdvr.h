@@ -100,6 +100,31 @@ int sdv_get_reducedpot(sdv_ctx* ctx, const double* twoereduced, const double* tw
 int sdv_get_reducedpot_dev(sdv_ctx* ctx, const void* twoereduced, const void* twoden, void* reducedpot,
                            void* stream);
 
+/* Pair potentials only: the convolution half of call_twoe_matel00
+ * (SD/coreproject.f90:1039-1081), filling twoereduced(totpoints, numspf,
+ * lowspf:highspf) without the matrix-element reduction. */
+int sdv_twoereduced(sdv_ctx* ctx, int lowspf, int highspf, const double* inspfs10, const double* inspfs20,
+                    double* twoereduced);
+int sdv_twoereduced_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs10, const void* inspfs20,
+                        void* twoereduced, void* stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream, the
+ * library's counterpart of the reference's twoematel.time.dat / zke.time.dat
+ * counters (SD/coreproject.f90:1114-1142,1499-1517).  sdv_profile(ctx, 1)
+ * clears and starts recording; sdv_profile_read synchronises and returns, per
+ * stage, the summed milliseconds and the launch count (arrays of
+ * SDV_NSTAGES). */
+#define SDV_STAGE_XFWD 0
+#define SDV_STAGE_YZ 1
+#define SDV_STAGE_XINV 2
+#define SDV_STAGE_MATEL_RHO 3
+#define SDV_STAGE_MATEL_GEMM 4
+#define SDV_STAGE_MATEL_REDUCE 5
+#define SDV_STAGE_KE 6
+#define SDV_NSTAGES 7
+int sdv_profile(sdv_ctx* ctx, int enable);
+int sdv_profile_read(sdv_ctx* ctx, double* ms, int* count);
+
 /* Inspection (tests): the kernel threed_two on its first octant, n^3 reals
  * K(i,j,k) for i,j,k in [0,n) (SD/sincproject.f90:182-187); the 1-D KE
  * stencil ketot(1-n:n-1) (SD/SineDVR.f90:32); the circulant edge M in use. */

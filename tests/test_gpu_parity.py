@@ -170,3 +170,22 @@ def test_twoe_matel_c1_41(orc, torch_dev):
     otm, otr = orc.twoe_matel(n, kern, phi, phi, M=81)
     assert relerr(tr, otr) < TOL64
     assert relerr(tm, otm) < TOL64
+
+
+def test_twoereduced_entry_and_profiling(torch_dev, c0):
+    """sdv_twoereduced_dev delivers the same pair potentials as
+    call_twoe_matel00, and the event profiler counts one launch per stage."""
+    from sincdvr import SincDVR
+    torch = torch_dev
+    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    say("twoereduced + profile")
+    s = SincDVR(n, dx, 4)
+    p = torch.from_numpy(np.ascontiguousarray(c0["phi_h"].T)).cuda()
+    _, tr = s.call_twoe_matel00(1, 4, p, p)
+    s.profile(True)
+    tr2 = s.twoereduced(1, 4, p)
+    prof = s.profile_read()
+    s.profile(False)
+    assert torch.equal(tr, tr2)
+    assert prof["xfwd"][1] == prof["yz"][1] == prof["xinv"][1] >= 1
+    assert prof["yz"][0] > 0

@@ -4,9 +4,12 @@
 // call_twoe_matel00, mult_ke, mult_reducedpot, get_reducedpot0's GEMM) on
 // device-resident data.  No computation of the operators happens on the host:
 // the host builds index lists (pair bookkeeping) and launches kernels.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "sdv_internal.h"
@@ -113,8 +116,20 @@ __global__ void k_assemble(int N, int nnn, const int2* __restrict__ rmap, const 
 
 using namespace sdv;
 
+// Index lists of one call shape, uploaded once and kept on the device: the
+// convolution items, and the row/column maps of the matrix-element assembly.
+struct IdxPlan {
+  std::vector<PairItem> items;
+  std::vector<int2> rows, rmap, cmap;
+  std::vector<int> cols;
+  void* dev = nullptr;
+  size_t off_items = 0, off_rows = 0, off_cols = 0, off_rmap = 0, off_cmap = 0;
+};
+
 struct sdv_ctx {
   sdv_config cfg;
+  Prof prof;
+  std::map<std::tuple<int, int, int, int>, IdxPlan> plans;  // (kind, low, high, sym)
   int n = 0, n3 = 0, N = 0;
   int precision = 0;
   int pair_batch = 8;
@@ -128,7 +143,7 @@ struct sdv_ctx {
     void* p = nullptr;
     size_t cap = 0;
   };
-  Buf work, scratch, idx, gout, h_in1, h_in2, h_out1, h_out2;
+  Buf work, scratch, gout, h_in1, h_in2, h_out1, h_out2;
 };
 
 static int ensure(sdv_ctx::Buf& b, size_t bytes) {
@@ -230,8 +245,13 @@ int sdv_free(sdv_ctx* c) {
   hipFree(c->ke);
   hipFree(c->invj);
   hipFree(c->sdiag);
-  for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->idx, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2})
+  for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2})
     hipFree(b->p);
+  for (auto& kv : c->plans) hipFree(kv.second.dev);
+  for (auto& r : c->prof.recs) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return SDV_OK;
@@ -258,17 +278,111 @@ int sdv_get_ke_stencil(sdv_ctx* c, double* ketot) {
   return SDV_OK;
 }
 
-// Run the convolution over a list of pair items, in batches sized to the
-// work buffer.
-static int run_pairs(sdv_ctx* c, const std::vector<PairItem>& items, const void* phi1, const void* phi2,
-                     int density_mode, void* out, hipStream_t st) {
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Upload a plan's index lists once; later calls with the same shape reuse them.
+static int upload_plan(IdxPlan& pl) {
+  pl.off_items = 0;
+  pl.off_rows = align256(pl.items.size() * sizeof(PairItem));
+  pl.off_cols = align256(pl.off_rows + pl.rows.size() * sizeof(int2));
+  pl.off_rmap = align256(pl.off_cols + pl.cols.size() * sizeof(int));
+  pl.off_cmap = align256(pl.off_rmap + pl.rmap.size() * sizeof(int2));
+  const size_t bytes = pl.off_cmap + pl.cmap.size() * sizeof(int2) + 16;
+  if (hipMalloc(&pl.dev, bytes) != hipSuccess) {
+    pl.dev = nullptr;
+    set_error("device allocation of the index plan failed");
+    return SDV_ERR_NOMEM;
+  }
+  char* d = (char*)pl.dev;
+  auto put = [&](size_t off, const void* src, size_t nb) -> hipError_t {
+    return nb ? hipMemcpy(d + off, src, nb, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  SDV_HIP_CHECK(put(pl.off_items, pl.items.data(), pl.items.size() * sizeof(PairItem)));
+  SDV_HIP_CHECK(put(pl.off_rows, pl.rows.data(), pl.rows.size() * sizeof(int2)));
+  SDV_HIP_CHECK(put(pl.off_cols, pl.cols.data(), pl.cols.size() * sizeof(int)));
+  SDV_HIP_CHECK(put(pl.off_rmap, pl.rmap.data(), pl.rmap.size() * sizeof(int2)));
+  SDV_HIP_CHECK(put(pl.off_cmap, pl.cmap.data(), pl.cmap.size() * sizeof(int2)));
+  return SDV_OK;
+}
+
+// Pair bookkeeping of call_twoe_matelxxx00 (SD/coreproject.f90:1039-1100):
+// twoereduced(:, 2a, 2b), 2a in [1,N], 2b in [low,high], column
+// (2a-1) + N (2b-low); twoematel(1a,1b,2a,2b).  With identical bra and ket
+// orbitals each unordered pair is convolved once; its mirror column gets
+// conj(V) (hermnorm) or V (cnorm), and rows (1a,1b) with 1a > 1b reuse the
+// density of (1b,1a) the same way.
+static int twoe_plan(sdv_ctx* c, int low, int high, bool sym, IdxPlan** out) {
+  auto key = std::make_tuple(1, low, high, sym ? 1 : 0);
+  auto it = c->plans.find(key);
+  if (it != c->plans.end()) {
+    *out = &it->second;
+    return SDV_OK;
+  }
+  const int N = c->N;
+  const int nnn = high - low + 1;
+  const bool herm = c->cfg.cnormflag == 0;
+  IdxPlan pl;
+  auto col = [&](int a2, int b2) { return (a2 - 1) + N * (b2 - low); };
+  pl.cmap.assign((size_t)N * nnn, int2{-1, 0});
+  for (int b2 = low; b2 <= high; ++b2)
+    for (int a2 = 1; a2 <= N; ++a2) {
+      const bool mirror_needed = sym && a2 != b2 && a2 >= low && a2 <= high;
+      if (mirror_needed && a2 > b2) continue;  // produced as the mirror of (b2, a2)
+      const int ci = (int)pl.items.size();
+      pl.items.push_back(PairItem{a2 - 1, b2 - 1, col(a2, b2), mirror_needed ? col(b2, a2) : -1});
+      pl.cmap[col(a2, b2)] = int2{ci, 0};
+      if (mirror_needed) pl.cmap[col(b2, a2)] = int2{ci, herm ? 1 : 0};
+    }
+  pl.rmap.resize((size_t)N * N);
+  if (sym) {
+    std::vector<int> ridx((size_t)N * N, -1);
+    for (int b1 = 0; b1 < N; ++b1)
+      for (int a1 = 0; a1 <= b1; ++a1) {
+        ridx[a1 + N * b1] = (int)pl.rows.size();
+        pl.rows.push_back(int2{a1, b1});
+      }
+    for (int b1 = 0; b1 < N; ++b1)
+      for (int a1 = 0; a1 < N; ++a1)
+        pl.rmap[a1 + N * b1] = a1 <= b1 ? int2{ridx[a1 + N * b1], 0} : int2{ridx[b1 + N * a1], herm ? 1 : 0};
+  } else {
+    for (int b1 = 0; b1 < N; ++b1)
+      for (int a1 = 0; a1 < N; ++a1) {
+        pl.rmap[a1 + N * b1] = int2{(int)pl.rows.size(), 0};
+        pl.rows.push_back(int2{a1, b1});
+      }
+  }
+  pl.cols.resize(pl.items.size());
+  for (size_t i = 0; i < pl.items.size(); ++i) pl.cols[i] = pl.items[i].col;
+  SDV_TRY(upload_plan(pl));
+  auto ins = c->plans.emplace(key, std::move(pl));
+  *out = &ins.first->second;
+  return SDV_OK;
+}
+
+static int density_plan(sdv_ctx* c, int allsize, IdxPlan** out) {
+  auto key = std::make_tuple(0, allsize, 0, 0);
+  auto it = c->plans.find(key);
+  if (it != c->plans.end()) {
+    *out = &it->second;
+    return SDV_OK;
+  }
+  IdxPlan pl;
+  pl.items.resize(allsize);
+  for (int i = 0; i < allsize; ++i) pl.items[i] = PairItem{i, 0, i, -1};
+  SDV_TRY(upload_plan(pl));
+  auto ins = c->plans.emplace(key, std::move(pl));
+  *out = &ins.first->second;
+  return SDV_OK;
+}
+
+// Run the convolution over a plan's items, in batches sized to the work buffer.
+static int run_pairs(sdv_ctx* c, const IdxPlan& pl, const void* phi1, const void* phi2, int density_mode, void* out,
+                     hipStream_t st) {
   const int cn = c->cfg.cnormflag;
-  const int np = (int)items.size();
+  const int np = (int)pl.items.size();
   if (np == 0) return SDV_OK;
   const int batch = std::min(np, c->pair_batch);
   SDV_TRY(ensure(c->work, conv_work_bytes(c->spec.g, batch, c->precision)));
-  SDV_TRY(ensure(c->idx, (size_t)np * sizeof(PairItem)));
-  SDV_HIP_CHECK(hipMemcpyAsync(c->idx.p, items.data(), (size_t)np * sizeof(PairItem), hipMemcpyHostToDevice, st));
   for (int p0 = 0; p0 < np; p0 += batch) {
     ConvArgsHost a;
     a.n = c->n;
@@ -276,12 +390,12 @@ static int run_pairs(sdv_ctx* c, const std::vector<PairItem>& items, const void*
     a.phi1 = phi1;
     a.phi2 = phi2;
     a.density_mode = density_mode;
-    a.conj_b = cn ? 0 : 1;
+    a.conj_b = (cn || density_mode) ? 0 : 1;
     a.mirror_conj = cn ? 0 : 1;
-    a.pairs_dev = (const PairItem*)c->idx.p + p0;
+    a.pairs_dev = (const PairItem*)((char*)pl.dev + pl.off_items) + p0;
     a.work = c->work.p;
     a.out = out;
-    SDV_TRY(conv_launch(c->spec, a, c->precision, st));
+    SDV_TRY(conv_launch(c->spec, a, c->precision, st, &c->prof));
   }
   return SDV_OK;
 }
@@ -296,9 +410,30 @@ int sdv_op_tinv_dev(sdv_ctx* c, const void* den, void* red, int allsize, int cir
     SDV_HIP_CHECK(hipMemsetAsync(red, 0, (size_t)c->n3 * allsize * sizeof(z64), st));
     return SDV_OK;
   }
-  std::vector<PairItem> items(allsize);
-  for (int i = 0; i < allsize; ++i) items[i] = PairItem{i, 0, i, -1};
-  return run_pairs(c, items, den, den, 1, red, st);
+  if (allsize == 0) return SDV_OK;
+  IdxPlan* pl = nullptr;
+  SDV_TRY(density_plan(c, allsize, &pl));
+  return run_pairs(c, *pl, den, den, 1, red, st);
+}
+
+static int check_twoe_args(sdv_ctx* c, int low, int high) {
+  SDV_REQUIRE(low >= 1 && high <= c->N, SDV_ERR_ARG, "programmer fail: lowspf/highspf out of range");
+  return SDV_OK;
+}
+
+int sdv_twoereduced_dev(sdv_ctx* c, int low, int high, const void* phi1, const void* phi2, void* twoereduced,
+                        void* stream) {
+  SDV_REQUIRE(c && phi1 && phi2 && twoereduced, SDV_ERR_ARG, "null argument");
+  if (high < low) return SDV_OK;
+  SDV_TRY(check_twoe_args(c, low, high));
+  hipStream_t st = (hipStream_t)stream;
+  if (c->cfg.notwoflag) {
+    SDV_HIP_CHECK(hipMemsetAsync(twoereduced, 0, (size_t)c->n3 * c->N * (high - low + 1) * sizeof(z64), st));
+    return SDV_OK;
+  }
+  IdxPlan* pl = nullptr;
+  SDV_TRY(twoe_plan(c, low, high, phi1 == phi2, &pl));
+  return run_pairs(c, *pl, phi1, phi2, 0, twoereduced, st);
 }
 
 int sdv_call_twoe_matel00_dev(sdv_ctx* c, int low, int high, const void* phi1, const void* phi2, void* twoematel,
@@ -307,93 +442,72 @@ int sdv_call_twoe_matel00_dev(sdv_ctx* c, int low, int high, const void* phi1, c
   const int N = c->N, n3 = c->n3;
   const int nnn = high - low + 1;
   if (nnn <= 0) return SDV_OK;  // SD/coreproject.f90:969-972
-  SDV_REQUIRE(low >= 1 && high <= N, SDV_ERR_ARG, "programmer fail: lowspf/highspf out of range");
+  SDV_TRY(check_twoe_args(c, low, high));
   hipStream_t st = (hipStream_t)stream;
-  const size_t tr_bytes = (size_t)n3 * N * nnn * sizeof(z64);
-  const size_t tm_bytes = (size_t)N * N * N * nnn * sizeof(z64);
   if (c->cfg.notwoflag) {  // :1019-1023
-    SDV_HIP_CHECK(hipMemsetAsync(twoereduced, 0, tr_bytes, st));
-    SDV_HIP_CHECK(hipMemsetAsync(twoematel, 0, tm_bytes, st));
+    SDV_HIP_CHECK(hipMemsetAsync(twoereduced, 0, (size_t)n3 * N * nnn * sizeof(z64), st));
+    SDV_HIP_CHECK(hipMemsetAsync(twoematel, 0, (size_t)N * N * N * nnn * sizeof(z64), st));
     return SDV_OK;
   }
   const bool sym = (phi1 == phi2);
-  const bool herm = c->cfg.cnormflag == 0;
-  // ---- pair bookkeeping: twoereduced(:, 2a, 2b) column (2a-1) + N (2b-low)
-  auto col = [&](int a2, int b2) { return (a2 - 1) + N * (b2 - low); };
-  std::vector<PairItem> items;
-  std::vector<int2> cmap((size_t)N * nnn, int2{-1, 0});
-  for (int b2 = low; b2 <= high; ++b2)
-    for (int a2 = 1; a2 <= N; ++a2) {
-      const bool mirror_needed = sym && a2 != b2 && a2 >= low && a2 <= high;
-      if (mirror_needed && a2 > b2) continue;  // produced as the mirror of (b2, a2)
-      const int ci = (int)items.size();
-      items.push_back(PairItem{a2 - 1, b2 - 1, col(a2, b2), mirror_needed ? col(b2, a2) : -1});
-      cmap[col(a2, b2)] = int2{ci, 0};
-      if (mirror_needed) cmap[col(b2, a2)] = int2{ci, herm ? 1 : 0};
-    }
-  SDV_TRY(run_pairs(c, items, phi1, phi2, 0, twoereduced, st));
-  // ---- rows of the reduction
-  std::vector<int2> rows;
-  std::vector<int2> rmap((size_t)N * N);
-  if (sym) {
-    std::vector<int> ridx((size_t)N * N, -1);
-    for (int b1 = 0; b1 < N; ++b1)
-      for (int a1 = 0; a1 <= b1; ++a1) {
-        ridx[a1 + N * b1] = (int)rows.size();
-        rows.push_back(int2{a1, b1});
-      }
-    for (int b1 = 0; b1 < N; ++b1)
-      for (int a1 = 0; a1 < N; ++a1)
-        rmap[a1 + N * b1] = a1 <= b1 ? int2{ridx[a1 + N * b1], 0} : int2{ridx[b1 + N * a1], herm ? 1 : 0};
-  } else {
-    for (int b1 = 0; b1 < N; ++b1)
-      for (int a1 = 0; a1 < N; ++a1) {
-        rmap[a1 + N * b1] = int2{(int)rows.size(), 0};
-        rows.push_back(int2{a1, b1});
-      }
-  }
-  std::vector<int> cols(items.size());
-  for (size_t i = 0; i < items.size(); ++i) cols[i] = items[i].col;
-  const int nrows = (int)rows.size(), ncols = (int)cols.size();
-  // index buffers live after the pair items in c->idx
-  const size_t off_items = items.size() * sizeof(PairItem);
-  const size_t off_rows = (off_items + 255) & ~(size_t)255;
-  const size_t off_cols = (off_rows + rows.size() * sizeof(int2) + 255) & ~(size_t)255;
-  const size_t off_rmap = (off_cols + cols.size() * sizeof(int) + 255) & ~(size_t)255;
-  const size_t off_cmap = (off_rmap + rmap.size() * sizeof(int2) + 255) & ~(size_t)255;
-  const size_t idx_bytes = off_cmap + cmap.size() * sizeof(int2);
-  // run_pairs already used c->idx for the items; grow keeps contents only if
-  // no realloc happens, so stage the matel indices in a separate region.
-  SDV_HIP_CHECK(hipStreamSynchronize(st));
-  SDV_TRY(ensure(c->idx, idx_bytes));
-  char* ib = (char*)c->idx.p;
-  SDV_HIP_CHECK(hipMemcpyAsync(ib + off_rows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-  SDV_HIP_CHECK(hipMemcpyAsync(ib + off_cols, cols.data(), cols.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  SDV_HIP_CHECK(hipMemcpyAsync(ib + off_rmap, rmap.data(), rmap.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-  SDV_HIP_CHECK(hipMemcpyAsync(ib + off_cmap, cmap.data(), cmap.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+  IdxPlan* pl = nullptr;
+  SDV_TRY(twoe_plan(c, low, high, sym, &pl));
+  SDV_TRY(run_pairs(c, *pl, phi1, phi2, 0, twoereduced, st));
+  const int nrows = (int)pl->rows.size(), ncols = (int)pl->cols.size();
   SDV_TRY(ensure(c->scratch, matel_scratch_bytes(n3, nrows, ncols)));
   SDV_TRY(ensure(c->gout, (size_t)(2 * nrows) * (2 * ncols) * sizeof(double)));
+  const char* ib = (const char*)pl->dev;
   MatelArgsHost m;
   m.n3 = n3;
   m.N = N;
   m.nrows = nrows;
-  m.rows_dev = (const int2*)(ib + off_rows);
+  m.rows_dev = (const int2*)(ib + pl->off_rows);
   m.ncols = ncols;
-  m.cols_dev = (const int*)(ib + off_cols);
+  m.cols_dev = (const int*)(ib + pl->off_cols);
   m.phi1 = phi1;
   m.phi2 = phi2;
-  m.conj_b = herm ? 1 : 0;
+  m.conj_b = c->cfg.cnormflag ? 0 : 1;
   m.tr = twoereduced;
   m.gout = (double*)c->gout.p;
   m.scratch = c->scratch.p;
-  SDV_TRY(matel_launch(m, st));
+  SDV_TRY(matel_launch(m, st, &c->prof));
   const long tot = (long)N * N * N * nnn;
+  hipEvent_t e0 = c->prof.begin(st);
   hipLaunchKernelGGL(k_assemble, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, N, nnn,
-                     (const int2*)(ib + off_rmap), (const int2*)(ib + off_cmap), (const double*)c->gout.p,
+                     (const int2*)(ib + pl->off_rmap), (const int2*)(ib + pl->off_cmap), (const double*)c->gout.p,
                      2 * ncols, (z64*)twoematel);
   SDV_HIP_CHECK(hipGetLastError());
-  // the index buffer is reused by the next call: finish before returning
-  SDV_HIP_CHECK(hipStreamSynchronize(st));
+  c->prof.end(SDV_STAGE_MATEL_REDUCE, e0, st);
+  return SDV_OK;
+}
+
+int sdv_profile(sdv_ctx* c, int enable) {
+  SDV_REQUIRE(c, SDV_ERR_ARG, "null context");
+  SDV_HIP_CHECK(hipDeviceSynchronize());
+  for (auto& r : c->prof.recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  c->prof.recs.clear();
+  c->prof.on = enable != 0;
+  return SDV_OK;
+}
+
+int sdv_profile_read(sdv_ctx* c, double* ms, int* count) {
+  SDV_REQUIRE(c && ms && count, SDV_ERR_ARG, "null argument");
+  for (int i = 0; i < SDV_NSTAGES; ++i) {
+    ms[i] = 0.0;
+    count[i] = 0;
+  }
+  for (auto& r : c->prof.recs) {
+    SDV_HIP_CHECK(hipEventSynchronize(r.b));
+    float t = 0.f;
+    SDV_HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
+    if (r.stage >= 0 && r.stage < SDV_NSTAGES) {
+      ms[r.stage] += t;
+      count[r.stage] += 1;
+    }
+  }
   return SDV_OK;
 }
 
@@ -408,7 +522,7 @@ int sdv_mult_ke_dev(sdv_ctx* c, const void* in, void* out, int howmany, void* st
   a.sdiag = c->sdiag;
   a.in = in;
   a.out = out;
-  return ke_launch(a, c->precision, (hipStream_t)stream);
+  return ke_launch(a, c->precision, (hipStream_t)stream, &c->prof);
 }
 
 int sdv_mult_reducedpot_dev(sdv_ctx* c, int first, int last, const void* inspfs, void* outspfs,
@@ -461,6 +575,25 @@ int sdv_call_twoe_matel00(sdv_ctx* c, int low, int high, const double* phi1, con
   SDV_TRY(sdv_call_twoe_matel00_dev(c, low, high, d1, d2, c->h_out2.p, c->h_out1.p, c->stream));
   SDV_HIP_CHECK(hipMemcpyAsync(tr, c->h_out1.p, trb, hipMemcpyDefault, c->stream));
   SDV_HIP_CHECK(hipMemcpyAsync(tm, c->h_out2.p, tmb, hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+int sdv_twoereduced(sdv_ctx* c, int low, int high, const double* phi1, const double* phi2, double* tr) {
+  SDV_REQUIRE(c && phi1 && phi2 && tr, SDV_ERR_ARG, "null argument");
+  const int N = c->N, nnn = high - low + 1;
+  if (nnn <= 0) return SDV_OK;
+  const size_t pb = (size_t)c->n3 * N * sizeof(z64);
+  const void *d1, *d2;
+  SDV_TRY(stage_in(c, c->h_in1, phi1, pb, &d1));
+  if (phi2 == phi1)
+    d2 = d1;
+  else
+    SDV_TRY(stage_in(c, c->h_in2, phi2, pb, &d2));
+  const size_t trb = (size_t)c->n3 * N * nnn * sizeof(z64);
+  SDV_TRY(ensure(c->h_out1, trb));
+  SDV_TRY(sdv_twoereduced_dev(c, low, high, d1, d2, c->h_out1.p, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(tr, c->h_out1.p, trb, hipMemcpyDefault, c->stream));
   SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
   return SDV_OK;
 }

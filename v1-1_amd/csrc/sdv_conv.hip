@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
 
 // ---------------------------------------------------------------- dispatch
 template <typename T, int P, int Q>
-static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st) {
+static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, Prof* prof) {
   constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
   KArgs<T> a;
   a.n = h.n;
@@ -427,12 +427,18 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st) {
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
   dim3 gX((a.n2 + LA - 1) / LA, h.npairs);
   dim3 gB(2 * R, h.npairs);
+  hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
   hipLaunchKernelGGL((k_xfwd<T, P, Q>), gX, dim3(256), shX, st, a);
   SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
+  e0 = prof ? prof->begin(st) : nullptr;
   hipLaunchKernelGGL((k_yz<T, P, Q>), gB, dim3(YZCfg<P, Q>::NT), shB, st, a);
   SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_YZ, e0, st);
+  e0 = prof ? prof->begin(st) : nullptr;
   hipLaunchKernelGGL((k_xinv<T, P, Q>), gX, dim3(256), shX, st, a);
   SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_XINV, e0, st);
   return SDV_OK;
 }
 
@@ -463,11 +469,11 @@ size_t conv_work_bytes(const Geom& g, int npairs, int precision) {
 }
 
 template <typename T>
-static int conv_dispatch(const Spectrum& s, const ConvArgsHost& h, hipStream_t st) {
+static int conv_dispatch(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, Prof* prof) {
   switch (s.g.m) {
 #define SDV_CASE(m_, P_, Q_) \
   case m_:                   \
-    return launch_pq<T, P_, Q_>(s, h, st);
+    return launch_pq<T, P_, Q_>(s, h, st, prof);
     SDV_CASE(9, 3, 3)
     SDV_CASE(12, 4, 3)
     SDV_CASE(16, 4, 4)
@@ -490,9 +496,9 @@ static int conv_dispatch(const Spectrum& s, const ConvArgsHost& h, hipStream_t s
   }
 }
 
-int conv_launch(const Spectrum& s, const ConvArgsHost& h, int precision, hipStream_t st) {
+int conv_launch(const Spectrum& s, const ConvArgsHost& h, int precision, hipStream_t st, Prof* prof) {
   if (h.npairs <= 0) return SDV_OK;
-  return precision == 0 ? conv_dispatch<double>(s, h, st) : conv_dispatch<float>(s, h, st);
+  return precision == 0 ? conv_dispatch<double>(s, h, st, prof) : conv_dispatch<float>(s, h, st, prof);
 }
 
 }  // namespace sdv

@@ -2,10 +2,34 @@
 // translation units.  Nothing here crosses the C-ABI.
 #pragma once
 #include <vector>
+#include <utility>
 
 #include "sdv_common.h"
 
 namespace sdv {
+
+// Per-kernel HIP-event timing on the launch stream (enabled by the caller
+// through sdv_profile).  Stage ids are the SDV_STAGE_* values of sincdvr.h.
+struct Prof {
+  bool on = false;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  hipEvent_t begin(hipStream_t st) {
+    hipEvent_t e = nullptr;
+    if (on && hipEventCreate(&e) == hipSuccess) (void)hipEventRecord(e, st);
+    return e;
+  }
+  void end(int stage, hipEvent_t a, hipStream_t st) {
+    if (!on || !a) return;
+    hipEvent_t b = nullptr;
+    if (hipEventCreate(&b) != hipSuccess) return;
+    (void)hipEventRecord(b, st);
+    recs.push_back(Rec{stage, a, b});
+  }
+};
 
 // One convolution work item: output column `col` of the (n^3, ncols) result
 // gets K * rho where rho = phi1[:,a] * CONJUGATE(phi2[:,b]) (orbital mode) or
@@ -47,7 +71,7 @@ struct ConvArgsHost {
 // Convolution pipeline: x-forward (pair density fused), y/z plane kernel with
 // the spectral multiply, x-inverse with output scatter.  precision 0 = FP64,
 // 1 = FP32 butterflies.
-int conv_launch(const Spectrum& s, const ConvArgsHost& a, int precision, hipStream_t st);
+int conv_launch(const Spectrum& s, const ConvArgsHost& a, int precision, hipStream_t st, Prof* prof);
 size_t conv_work_bytes(const Geom& g, int npairs, int precision);
 
 // Kernel construction (tooth + placement + spectrum), FP64.
@@ -69,7 +93,7 @@ struct MatelArgsHost {
   void* scratch;             // matel_scratch_bytes()
 };
 size_t matel_scratch_bytes(int n3, int nrows, int ncols);
-int matel_launch(const MatelArgsHost& a, hipStream_t st);
+int matel_launch(const MatelArgsHost& a, hipStream_t st, Prof* prof);
 
 // Kinetic apply
 struct KeArgsHost {
@@ -80,7 +104,7 @@ struct KeArgsHost {
   const void* in;            // z64 (n^3, howmany)
   void* out;
 };
-int ke_launch(const KeArgsHost& a, int precision, hipStream_t st);
+int ke_launch(const KeArgsHost& a, int precision, hipStream_t st, Prof* prof);
 
 // Mean field
 int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, hipStream_t st);

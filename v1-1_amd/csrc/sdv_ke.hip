@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(kKeThreads) k_ke_axis(int n, const double* __r
   }
 }
 
-int ke_launch(const KeArgsHost& a, int precision, hipStream_t st) {
+int ke_launch(const KeArgsHost& a, int precision, hipStream_t st, Prof* prof) {
   (void)precision;
   if (a.howmany <= 0) return SDV_OK;
   const int n = a.n;
@@ -114,6 +114,7 @@ int ke_launch(const KeArgsHost& a, int precision, hipStream_t st) {
   const int la0 = 32;
   const size_t sh0 = ((size_t)((2 * n - 1 + 1) & ~1)) * sizeof(double) + (size_t)la0 * n * sizeof(z64);
   SDV_REQUIRE(sh0 <= 64 * 1024, SDV_ERR_UNSUPPORTED, "grid edge too large for the KE tile");
+  hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
   const z64* ij = (const z64*)a.invj;
   const z64* sd = (const z64*)a.sdiag;
   {
@@ -134,6 +135,7 @@ int ke_launch(const KeArgsHost& a, int precision, hipStream_t st) {
                        (const z64*)a.in, (z64*)a.out, la0);
   }
   SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_KE, e0, st);
   return SDV_OK;
 }
 

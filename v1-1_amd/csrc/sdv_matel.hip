@@ -109,7 +109,7 @@ size_t matel_scratch_bytes(int n3, int nrows, int ncols) {
   return (size_t)nrows * n3 * sizeof(z64) + (size_t)ks * (2 * nrows) * (2 * ncols) * sizeof(double) + 256;
 }
 
-int matel_launch(const MatelArgsHost& a, hipStream_t st) {
+int matel_launch(const MatelArgsHost& a, hipStream_t st, Prof* prof) {
   if (a.nrows <= 0 || a.ncols <= 0) return SDV_OK;
   const int tilesR = (2 * a.nrows + kTM - 1) / kTM, tilesC = (2 * a.ncols + kTN - 1) / kTN;
   const int ks = ksplit_for(tilesR * tilesC);
@@ -118,15 +118,21 @@ int matel_launch(const MatelArgsHost& a, hipStream_t st) {
   const int ksplit = (a.n3 + kchunk - 1) / kchunk;
   z64* D = (z64*)a.scratch;
   double* part = (double*)((char*)a.scratch + (size_t)a.nrows * a.n3 * sizeof(z64));
+  hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
   hipLaunchKernelGGL(k_form_rho, dim3(256, a.nrows), dim3(256), 0, st, (const z64*)a.phi1, (const z64*)a.phi2,
                      a.rows_dev, a.n3, a.conj_b, D);
   SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_MATEL_RHO, e0, st);
+  e0 = prof ? prof->begin(st) : nullptr;
   hipLaunchKernelGGL(k_gemm_tn, dim3(tilesC, tilesR, ksplit), dim3(256), 0, st, D, (const z64*)a.tr, a.cols_dev,
                      a.nrows, a.ncols, a.n3, kchunk, part);
   SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_MATEL_GEMM, e0, st);
+  e0 = prof ? prof->begin(st) : nullptr;
   const long len = (long)(2 * a.nrows) * (2 * a.ncols);
   hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, ksplit, len, a.gout);
   SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_MATEL_REDUCE, e0, st);
   return SDV_OK;
 }
 

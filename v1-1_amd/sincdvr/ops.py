@@ -152,6 +152,35 @@ class SincDVR:
                                                    _hptr(tr)))
         return tm, tr
 
+    def twoereduced(self, lowspf, highspf, inspfs10, inspfs20=None, out=None):
+        """Pair potentials only (the convolution half of call_twoe_matel00)."""
+        N = self.numspf
+        nnn = highspf - lowspf + 1
+        same = inspfs20 is None or inspfs20 is inspfs10
+        if _is_torch(inspfs10):
+            p1 = inspfs10.contiguous()
+            p2 = p1 if same else inspfs20.contiguous()
+            tr = out if out is not None else torch.zeros((max(nnn, 0), N, self.totpoints), dtype=torch.complex128,
+                                                         device=p1.device)
+            _lib.check(self._lib.sdv_twoereduced_dev(self._h, lowspf, highspf, _dptr(p1), _dptr(p2), _dptr(tr),
+                                                     _stream()))
+            return tr
+        p1 = np.asfortranarray(inspfs10, dtype=np.complex128)
+        p2 = p1 if same else np.asfortranarray(inspfs20, dtype=np.complex128)
+        tr = np.zeros((self.totpoints, N, max(nnn, 0)), dtype=np.complex128, order="F")
+        _lib.check(self._lib.sdv_twoereduced(self._h, lowspf, highspf, _hptr(p1), _hptr(p2), _hptr(tr)))
+        return tr
+
+    def profile(self, enable=True):
+        _lib.check(self._lib.sdv_profile(self._h, int(enable)))
+
+    def profile_read(self):
+        """{stage: (total ms, launches)} from HIP events on the launch stream."""
+        ms = (ctypes.c_double * len(_lib.STAGES))()
+        cnt = (ctypes.c_int * len(_lib.STAGES))()
+        _lib.check(self._lib.sdv_profile_read(self._h, ms, cnt))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.STAGES)}
+
     # ---------------------------------------------------------------- mult_ke
     def mult_ke(self, inp):
         """SD/coreproject.f90:2104-2116."""
