@@ -117,7 +117,7 @@ int sdv_twoereduced_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspf
 #define SDV_STAGE_XFWD 0
 #define SDV_STAGE_YZ 1
 #define SDV_STAGE_XINV 2
-#define SDV_STAGE_MATEL_RHO 3
+#define SDV_STAGE_MATEL_ASSEMBLE 3
 #define SDV_STAGE_MATEL_GEMM 4
 #define SDV_STAGE_MATEL_REDUCE 5
 #define SDV_STAGE_KE 6

@@ -189,3 +189,20 @@ def test_twoereduced_entry_and_profiling(torch_dev, c0):
     assert torch.equal(tr, tr2)
     assert prof["xfwd"][1] == prof["yz"][1] == prof["xinv"][1] >= 1
     assert prof["yz"][0] > 0
+
+
+@pytest.mark.parametrize("lowhigh", [(1, 16), (3, 9)])
+def test_twoe_matel_16_orbitals_small_grid(orc, torch_dev, lowhigh):
+    """16 orbitals (136 unique pairs: the 81^3 workload's reduction tiling,
+    including the symmetric upper-triangle split-K sizing) on a 9^3 grid."""
+    from sincdvr import SincDVR, inputs
+    n, dx, N = 9, 0.5, 16
+    phi = inputs.gaussian_orbitals(n, dx, N, 2, seed=77)
+    low, high = lowhigh
+    say("call_twoe_matel00 n=9 N=16 low=%d high=%d" % (low, high))
+    s = SincDVR(n, dx, N)
+    tm, tr = s.call_twoe_matel00(low, high, phi, phi)
+    kern = orc.coulomb_kernel(n, dx)
+    otm, otr = orc.twoe_matel(n, kern, phi, phi, low=low, high=high)
+    assert relerr(tr, otr) < TOL64
+    assert relerr(tm, otm) < TOL64

@@ -132,7 +132,7 @@ struct sdv_ctx {
   std::map<std::tuple<int, int, int, int>, IdxPlan> plans;  // (kind, low, high, sym)
   int n = 0, n3 = 0, N = 0;
   int precision = 0;
-  int pair_batch = 8;
+  int pair_batch = 32;
   Spectrum spec;
   double* ke = nullptr;
   z64* invj = nullptr;
@@ -454,7 +454,9 @@ int sdv_call_twoe_matel00_dev(sdv_ctx* c, int low, int high, const void* phi1, c
   SDV_TRY(twoe_plan(c, low, high, sym, &pl));
   SDV_TRY(run_pairs(c, *pl, phi1, phi2, 0, twoereduced, st));
   const int nrows = (int)pl->rows.size(), ncols = (int)pl->cols.size();
-  SDV_TRY(ensure(c->scratch, matel_scratch_bytes(n3, nrows, ncols)));
+  const int msym = (sym && low == 1 && high == N) ? 1 : 0;
+  const size_t mbytes = matel_scratch_bytes(n3, nrows, ncols, msym);
+  SDV_TRY(ensure(c->scratch, mbytes));
   SDV_TRY(ensure(c->gout, (size_t)(2 * nrows) * (2 * ncols) * sizeof(double)));
   const char* ib = (const char*)pl->dev;
   MatelArgsHost m;
@@ -468,8 +470,10 @@ int sdv_call_twoe_matel00_dev(sdv_ctx* c, int low, int high, const void* phi1, c
   m.phi2 = phi2;
   m.conj_b = c->cfg.cnormflag ? 0 : 1;
   m.tr = twoereduced;
+  m.symmetric = msym;
   m.gout = (double*)c->gout.p;
   m.scratch = c->scratch.p;
+  m.scratch_bytes = c->scratch.cap;
   SDV_TRY(matel_launch(m, st, &c->prof));
   const long tot = (long)N * N * N * nnn;
   hipEvent_t e0 = c->prof.begin(st);
@@ -477,7 +481,7 @@ int sdv_call_twoe_matel00_dev(sdv_ctx* c, int low, int high, const void* phi1, c
                      (const int2*)(ib + pl->off_rmap), (const int2*)(ib + pl->off_cmap), (const double*)c->gout.p,
                      2 * ncols, (z64*)twoematel);
   SDV_HIP_CHECK(hipGetLastError());
-  c->prof.end(SDV_STAGE_MATEL_REDUCE, e0, st);
+  c->prof.end(SDV_STAGE_MATEL_ASSEMBLE, e0, st);
   return SDV_OK;
 }
 

@@ -31,6 +31,26 @@ __device__ __forceinline__ cplx<T> cvt(z64 v) {
   return mk((T)v.re, (T)v.im);
 }
 
+// Walks e = t, t + step, ... over a row-major (rows x width) array without a
+// per-element integer division: (row, col) advance incrementally.
+struct RowCol {
+  int row, col, drow, dcol, width;
+  __device__ __forceinline__ RowCol(int t, int step, int w) : width(w) {
+    row = t / w;
+    col = t - row * w;
+    drow = step / w;
+    dcol = step - drow * w;
+  }
+  __device__ __forceinline__ void next() {
+    col += dcol;
+    row += drow;
+    if (col >= width) {
+      col -= width;
+      ++row;
+    }
+  }
+};
+
 template <typename T>
 struct KArgs {
   int n, n2, n3, npairs;
@@ -42,7 +62,7 @@ struct KArgs {
   z64* out;
   const cplx<T>* tw;
   const T* khat;
-  int lds_tw2, lds_y;  // runtime LDS offsets (elements) of the 2-D twiddle table and workspace
+  int lds_tw2, lds_y, lds_z2, lds_in;  // runtime LDS offsets (elements) of the k_yz tables and workspaces
 };
 
 template <int P, int Q>
@@ -55,7 +75,8 @@ template <int P, int Q>
 struct YZCfg {
   static constexpr int R = P * Q;
   static constexpr int MX = P > Q ? P : Q;
-  static constexpr int NT = ((R * MX + 63) / 64) * 64;
+  static constexpr int H = (R + 1) / 2;                       // rows per z half-pass
+  static constexpr int NT = ((2 * H * MX + 63) / 64) * 64;    // two channel groups of H*MX tasks
 };
 
 // ---------------------------------------------------------------- x forward
@@ -90,7 +111,8 @@ __global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
   for (int i = tid; i < M; i += 256) tw[i] = a.tw[i];
   const z64* pa = a.phi1 + (size_t)it.a * a.n3 + (size_t)l0 * n;
   const z64* pb = a.phi2 + (size_t)it.b * a.n3 + (size_t)l0 * n;
-  for (int e = tid; e < nl * n; e += 256) {
+  RowCol rc(tid, 256, n);
+  for (int e = tid; e < nl * n; e += 256, rc.next()) {
     z64 v;
     if (a.density_mode) {
       v = pa[e];
@@ -99,8 +121,7 @@ __global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
       if (a.conj_b) w.im = -w.im;
       v = u * w;
     }
-    const int line = e / n, x = e - line * n;
-    buf[line * pitch + x] = cvt<T>(v);
+    buf[rc.row * pitch + rc.col] = cvt<T>(v);
   }
   __syncthreads();
   const int line = tid % LA, j2 = tid / LA;
@@ -136,17 +157,29 @@ __global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
 }
 
 // ------------------------------------------------------------- y/z planes
-// LDS: tw[M] = w_M^k ; T2[a*P + b] = w_m^(a b) (a < Q, b < P) ; Y = m x pitch
-// workspace.  The offsets of T2 and Y are runtime values, so the compiler
-// cannot prove the twiddle tables distinct from the workspace and reloads
-// them after each workspace store instead of pinning them in VGPRs.
+// One (pair, kx) plane per workgroup.  LDS (elements of cplx<T>):
+//   tw[M]  = w_M^k ;  T2[a*P + b] = w_m^(a b) (a < Q, b < P)
+//   Y      = m rows x pitch : the plane, then its y-transform [ky-pos][z]
+//   Z2     = H rows x pitch : workspace of the odd z channel
+//   IN     = n x n          : (FP32 only) the input plane, kept for both ky channels
+// The z pass runs over two halves of the ky rows; within a half the even and
+// odd z channels run at the same time on two thread groups (even channel in
+// place in Y, odd channel in Z2), so no per-thread state crosses a phase.
+// FP64 has no room for IN: it re-reads the plane for the odd ky channel and
+// parks the even channel's output in the plane's global slot meanwhile.
+// The LDS offsets are runtime values, so the compiler cannot prove the
+// twiddle tables distinct from the workspaces and does not pin hoisted
+// twiddles in VGPRs across phases.
 template <typename T, int P, int Q>
 __global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
-  constexpr int R = P * Q, NT = YZCfg<P, Q>::NT, pitch = R | 1, M = 2 * R;
+  constexpr int R = P * Q, NT = YZCfg<P, Q>::NT, H = YZCfg<P, Q>::H, G = NT / 2, pitch = R | 1, M = 2 * R;
+  constexpr bool KEEP = sizeof(T) == 4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* T2 = tw + a.lds_tw2;
   cplx<T>* Y = tw + a.lds_y;
+  cplx<T>* Z2 = tw + a.lds_z2;
+  cplx<T>* IN = tw + a.lds_in;
   const int tid = threadIdx.x;
   const int n = a.n, n2 = a.n2;
   const int plane = blockIdx.x;
@@ -161,56 +194,53 @@ __global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
     const int aa = i / P, bb = i - aa * P;
     T2[i] = a.tw[(2 * aa * bb) % M];
   }
+  const int yz = tid % n, yj = tid / n;            // y-line tasks: (z, j2) / (z, k1)
+  const bool ya = yj < Q, yb = yj < P;
+  const int cz = tid / G, tg = tid - cz * G;       // z tasks: channel group, index in group
+  const int zrr = tg % H, zj = tg / H;             // (row in half, j2) / (row in half, k1)
+  if (KEEP) {  // the input plane stays in LDS for both ky channels
+    RowCol rc(tid, NT, n);
+    for (int e = tid; e < n2; e += NT, rc.next()) IN[rc.row * n + rc.col] = base[e];
+  }
 
-  const int yz0 = tid % n, yj0 = tid / n;  // y-line tasks: (z, j2) / (z, k1)
-  const int zr0 = tid % R, zj0 = tid / R;  // z-line tasks: (row, j2) / (row, k1)
-
-  // acc holds one ky channel's y-inverse output.  The cy = 0 result is parked
-  // in the plane's own global slot while cy = 1 runs (after the input has been
-  // staged), so no accumulator is live across the z pass.
-  cplx<T> acc[P];
+  cplx<T> acc[P], acc0[KEEP ? P : 1];
 #pragma unroll 1
   for (int cy = 0; cy < 2; ++cy) {
-    // Opaque copies of the task coordinates: index arithmetic derived from
-    // them is recomputed in each iteration instead of being hoisted and held
-    // in VGPRs across the z pass.
-    int yz = yz0, yj = yj0, zr = zr0, zj = zj0;
-    asm volatile("" : "+v"(yz), "+v"(yj), "+v"(zr), "+v"(zj));
-    const bool ya = yj < Q, yb = yj < P, za = zj < Q, zb = zj < P;
-    // stage the plane, layout [z][y]
     __syncthreads();
-    for (int e = tid; e < n2; e += NT) {
-      const int z = e / n, y = e - z * n;
-      Y[z * pitch + y] = base[e];
+    if (!KEEP) {  // stage the plane, layout [z][y]
+      RowCol rc(tid, NT, n);
+      for (int e = tid; e < n2; e += NT, rc.next()) Y[rc.row * pitch + rc.col] = base[e];
+      __syncthreads();
     }
-    __syncthreads();
     cplx<T> v[P];
     if (ya) {
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
         const int y = j1 * Q + yj;
-        cplx<T> u = (y < n) ? Y[yz * pitch + y] : mk((T)0, (T)0);
+        cplx<T> u = (y < n) ? (KEEP ? IN[yz * n + y] : Y[yz * pitch + y]) : mk((T)0, (T)0);
         v[j1] = cy ? u * tw[y] : u;
       }
     }
-    if (cy == 1) {  // park the cy = 0 output in global memory
-      __syncthreads();
-      if (ya) {
+    if (cy == 1) {
+      if (KEEP) {
 #pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) {
-          const int y = j1 * Q + yj;
-          if (y < n) Y[yz * pitch + y] = acc[j1];
+        for (int j1 = 0; j1 < P; ++j1) acc0[j1] = acc[j1];
+      } else {  // park the cy = 0 output in the plane's global slot
+        __syncthreads();
+        if (ya) {
+#pragma unroll
+          for (int j1 = 0; j1 < P; ++j1) {
+            const int y = j1 * Q + yj;
+            if (y < n) Y[yz * pitch + y] = acc[j1];
+          }
         }
-      }
-      __syncthreads();
-      for (int e = tid; e < n2; e += NT) {
-        const int z = e / n, y = e - z * n;
-        base[e] = Y[z * pitch + y];
+        __syncthreads();
+        RowCol rc(tid, NT, n);
+        for (int e = tid; e < n2; e += NT, rc.next()) base[e] = Y[rc.row * pitch + rc.col];
       }
     }
     __syncthreads();
-    // y forward, step 1 -> layout [y-position][z]
-    if (ya) {
+    if (ya) {  // y forward, step 1 -> layout [y-position][z]
       DFT<T, P, false>::run(v);
 #pragma unroll
       for (int k1 = 0; k1 < P; ++k1) Y[(k1 * Q + yj) * pitch + yz] = v[k1] * T2[yj * P + k1];
@@ -224,72 +254,65 @@ __global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
 #pragma unroll
       for (int k2 = 0; k2 < Q; ++k2) Y[(yj * Q + k2) * pitch + yz] = u[k2];
     }
-    __syncthreads();
-    // z pass on row zr (frequency ky), both z channels.  Step 1 of both
-    // channels runs up front; channel 1's result waits in registers.
-    const int k1y = zr / Q, k2y = zr - (zr / Q) * Q;
-    const int ky = 2 * (k1y + P * k2y) + cy;
-    const int kky = min(ky, M - ky);
-    cplx<T> w1[P], zout[P];
-    if (za) {
+    // z pass: two halves of the ky rows, both z channels concurrently
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      const int r0 = h * H;
+      const int nh = min(H, R - r0);
+      const int r = r0 + zrr;
+      const bool rowok = zrr < nh;
+      cplx<T>* zrow = cz ? Z2 + zrr * pitch : Y + r * pitch;
+      const int k1y = r / Q, k2y = r - (r / Q) * Q;
+      const int ky = 2 * (k1y + P * k2y) + cy;
+      const int kky = min(ky, M - ky);
+      __syncthreads();
+      if (rowok && zj < Q) {  // step 1: both groups read row r of Y, then write their channel
 #pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) {
-        const int z = j1 * Q + zj;
-        const cplx<T> u = (z < n) ? Y[zr * pitch + z] : mk((T)0, (T)0);
-        v[j1] = u;
-        w1[j1] = u * tw[z];
-      }
-      DFT<T, P, false>::run(v);
-      DFT<T, P, false>::run(w1);
-#pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) {
-        const cplx<T> t = T2[zj * P + k1];
-        Y[zr * pitch + k1 * Q + zj] = v[k1] * t;
-        w1[k1] = w1[k1] * t;
-      }
-    }
-#pragma unroll
-    for (int cz = 0; cz < 2; ++cz) {
-      if (cz == 1 && za) {
-#pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) Y[zr * pitch + k1 * Q + zj] = w1[k1];
+        for (int j1 = 0; j1 < P; ++j1) {
+          const int z = j1 * Q + zj;
+          const cplx<T> u = (z < n) ? Y[r * pitch + z] : mk((T)0, (T)0);
+          v[j1] = cz ? u * tw[z] : u;
+        }
       }
       __syncthreads();
-      if (zb) {
+      if (rowok && zj < Q) {
+        DFT<T, P, false>::run(v);
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) zrow[k1 * Q + zj] = v[k1] * T2[zj * P + k1];
+      }
+      __syncthreads();
+      if (rowok && zj < P) {  // step 2 + spectral multiply + inverse step 1
         cplx<T> u[Q];
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) u[j2] = Y[zr * pitch + zj * Q + j2];
+        for (int j2 = 0; j2 < Q; ++j2) u[j2] = zrow[zj * Q + j2];
         DFT<T, Q, false>::run(u);
+        const T* khr = kh + kky;
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) {
           const int kz = 2 * (zj + P * k2) + cz;
           const int kkz = min(kz, M - kz);
-          u[k2] = scale(u[k2], kh[kky + (R + 1) * kkz]);
+          u[k2] = scale(u[k2], khr[(R + 1) * kkz]);
         }
         DFT<T, Q, true>::run(u);
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) Y[zr * pitch + zj * Q + j2] = u[j2] * conj(T2[j2 * P + zj]);
+        for (int j2 = 0; j2 < Q; ++j2) zrow[zj * Q + j2] = u[j2] * conj(T2[j2 * P + zj]);
       }
       __syncthreads();
-      if (za) {
+      if (rowok && zj < Q) {  // inverse step 2; the odd channel carries w_M^-z
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) v[k1] = Y[zr * pitch + k1 * Q + zj];
+        for (int k1 = 0; k1 < P; ++k1) v[k1] = zrow[k1 * Q + zj];
         DFT<T, P, true>::run(v);
-        if (cz == 0) {
 #pragma unroll
-          for (int j1 = 0; j1 < P; ++j1) zout[j1] = v[j1];
-        } else {
-#pragma unroll
-          for (int j1 = 0; j1 < P; ++j1) zout[j1] = zout[j1] + v[j1] * conj(tw[j1 * Q + zj]);
+        for (int j1 = 0; j1 < P; ++j1) {
+          const int z = j1 * Q + zj;
+          zrow[z] = cz ? v[j1] * conj(tw[z]) : v[j1];
         }
       }
-      // the next channel's writes touch exactly the positions this task read
-    }
-    if (za) {
-#pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) {
-        const int z = j1 * Q + zj;
-        if (z < n) Y[zr * pitch + z] = zout[j1];
+      __syncthreads();
+      RowCol rc(tid, NT, n);
+      for (int e = tid; e < nh * n; e += NT, rc.next()) {  // even + odd channel
+        cplx<T>* d = Y + (r0 + rc.row) * pitch + rc.col;
+        *d = *d + Z2[rc.row * pitch + rc.col];
       }
     }
     __syncthreads();
@@ -310,27 +333,31 @@ __global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
       for (int j1 = 0; j1 < P; ++j1) acc[j1] = cy ? v[j1] * conj(tw[j1 * Q + yj]) : v[j1];
     }
   }
-  // output = parked cy = 0 part + cy = 1 part
-  const int yz = yz0, yj = yj0;
-  const bool ya = yj < Q;
+  // output = cy = 0 part (registers or parked) + cy = 1 part
   __syncthreads();
-  for (int e = tid; e < n2; e += NT) {
-    const int z = e / n, y = e - z * n;
-    Y[z * pitch + y] = base[e];
-  }
-  __syncthreads();
-  if (ya) {
+  if (KEEP) {
+    if (ya) {
 #pragma unroll
-    for (int j1 = 0; j1 < P; ++j1) {
-      const int y = j1 * Q + yj;
-      if (y < n) Y[yz * pitch + y] = Y[yz * pitch + y] + acc[j1];
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int y = j1 * Q + yj;
+        if (y < n) Y[yz * pitch + y] = acc0[j1] + acc[j1];
+      }
+    }
+  } else {
+    RowCol rc(tid, NT, n);
+    for (int e = tid; e < n2; e += NT, rc.next()) Y[rc.row * pitch + rc.col] = base[e];
+    __syncthreads();
+    if (ya) {
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int y = j1 * Q + yj;
+        if (y < n) Y[yz * pitch + y] = Y[yz * pitch + y] + acc[j1];
+      }
     }
   }
   __syncthreads();
-  for (int e = tid; e < n2; e += NT) {
-    const int z = e / n, y = e - z * n;
-    base[e] = Y[z * pitch + y];
-  }
+  RowCol rc(tid, NT, n);
+  for (int e = tid; e < n2; e += NT, rc.next()) base[e] = Y[rc.row * pitch + rc.col];
 }
 
 // ---------------------------------------------------------------- x inverse
@@ -353,10 +380,8 @@ __global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
   for (int j = 0; j < P; ++j) acc[j] = mk((T)0, (T)0);
   for (int c = 0; c < 2; ++c) {
     const cplx<T>* src = a.X + ((size_t)pidx * 2 * R + (size_t)c * R) * a.n2 + l0;
-    for (int e = tid; e < R * nl; e += 256) {
-      const int pos = e / nl, ln = e - pos * nl;
-      buf[ln * pitch + pos] = src[(size_t)pos * a.n2 + ln];
-    }
+    RowCol rc(tid, 256, nl);
+    for (int e = tid; e < R * nl; e += 256, rc.next()) buf[rc.col * pitch + rc.row] = src[(size_t)rc.row * a.n2 + rc.col];
     __syncthreads();
     if (jj < P && line < nl) {  // inverse step 1: task (line, k1)
       cplx<T> u[Q];
@@ -388,9 +413,9 @@ __global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
   // twoereduced(:, col) and the mirror column (SD/coreproject.f90:521,528)
   z64* o = a.out + (size_t)it.col * a.n3 + (size_t)l0 * n;
   z64* om = it.mcol >= 0 ? a.out + (size_t)it.mcol * a.n3 + (size_t)l0 * n : nullptr;
-  for (int e = tid; e < nl * n; e += 256) {
-    const int ln = e / n, x = e - ln * n;
-    const cplx<T> v = buf[ln * pitch + x];
+  RowCol rc(tid, 256, n);
+  for (int e = tid; e < nl * n; e += 256, rc.next()) {
+    const cplx<T> v = buf[rc.row * pitch + rc.col];
     z64 d = mk((double)v.re, (double)v.im);
     o[e] = d;
     if (om) {
@@ -421,8 +446,10 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   a.khat = (const T*)(sizeof(T) == 8 ? (const void*)s.khat64 : (const void*)s.khat32);
   const size_t shX = (size_t)(M + LA * pitch) * sizeof(cplx<T>);
   a.lds_tw2 = M;
-  a.lds_y = M + R + ((M + R) & 1);  // keep the workspace 16-byte aligned for FP32
-  const size_t shB = (size_t)(a.lds_y + R * pitch) * sizeof(cplx<T>);
+  a.lds_y = M + R + ((M + R) & 1);  // keep the workspaces 16-byte aligned for FP32
+  a.lds_z2 = a.lds_y + R * pitch + ((R * pitch) & 1);
+  a.lds_in = a.lds_z2 + YZCfg<P, Q>::H * pitch + ((YZCfg<P, Q>::H * pitch) & 1);
+  const size_t shB = (size_t)(a.lds_in + (sizeof(T) == 4 ? a.n2 : 0)) * sizeof(cplx<T>);
   SDV_REQUIRE(shB <= 160 * 1024, SDV_ERR_UNSUPPORTED, "y/z plane does not fit in LDS");
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
   dim3 gX((a.n2 + LA - 1) / LA, h.npairs);

@@ -89,10 +89,12 @@ struct MatelArgsHost {
   const void* phi2;
   int conj_b;
   const void* tr;            // z64 twoereduced base
+  int symmetric;             // rows and cols are the same pair list
   double* gout;              // (2*nrows) x (2*ncols) real products, row-major
   void* scratch;             // matel_scratch_bytes()
+  size_t scratch_bytes;
 };
-size_t matel_scratch_bytes(int n3, int nrows, int ncols);
+size_t matel_scratch_bytes(int n3, int nrows, int ncols, int symmetric);
 int matel_launch(const MatelArgsHost& a, hipStream_t st, Prof* prof);
 
 // Kinetic apply

@@ -10,42 +10,66 @@
 // four real products, including the mirrored/conjugated pairs.  Reductions
 // over the split-K partials run in a fixed order (results do not depend on
 // scheduling).
+//
+// The GEMM runs on FP64 MFMA (v_mfma_f64_16x16x4f64); pair densities are
+// formed from the orbitals while the A tile is staged, never stored.  When the
+// row and column pair sets coincide (identical bra/ket orbitals, full
+// lowspf..highspf range) G is symmetric: sum rho_P V_Q = sum rho_Q V_P for the
+// real even Coulomb kernel, which makes the real product matrix symmetric, so
+// only the upper-triangle tiles are computed.
 #include "sdv_internal.h"
 
 namespace sdv {
 
 constexpr int kTM = 64, kTN = 64, kKC = 16;
 
-__global__ void k_form_rho(const z64* __restrict__ phi1, const z64* __restrict__ phi2, const int2* __restrict__ rows,
-                           int n3, int conj_b, z64* __restrict__ D) {
-  const int r = blockIdx.y;
-  const int2 ab = rows[r];
-  const z64* pa = phi1 + (size_t)ab.x * n3;
-  const z64* pb = phi2 + (size_t)ab.y * n3;
-  z64* d = D + (size_t)r * n3;
-  for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < n3; x += gridDim.x * blockDim.x) {
-    z64 w = pb[x];
-    if (conj_b) w.im = -w.im;
-    d[x] = pa[x] * w;
-  }
-}
+typedef double d4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256) k_gemm_tn(const z64* __restrict__ D, const z64* __restrict__ tr,
-                                                  const int* __restrict__ cols, int nrows, int ncols, int n3,
-                                                  int kchunk, double* __restrict__ part) {
+// 64 x 64 real output tile per workgroup (4 waves, 32 x 32 each = 2 x 2
+// MFMA 16x16 tiles), kKC grid points per LDS stage.  MFMA f64 16x16x4
+// operands: lane l holds A[l%16][l/16] and B[l/16][l%16]; the accumulator
+// holds D[4*v + l/16][l%16] in element v (checked on gfx950 with a probe).
+__global__ void __launch_bounds__(256) k_gemm_mfma(const z64* __restrict__ phi1, const z64* __restrict__ phi2,
+                                                    const int2* __restrict__ rows, int conj_b,
+                                                    const z64* __restrict__ tr, const int* __restrict__ cols,
+                                                    int nrows, int ncols, int n3, int kchunk, int sym, int tiles_c,
+                                                    double* __restrict__ part) {
   __shared__ double As[kKC][kTM + 2];
   __shared__ double Bs[kKC][kTN + 2];
   const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  int by, bx;
+  if (sym) {  // upper-triangle tile t -> (by, bx), bx >= by
+    int t = blockIdx.x;
+    by = 0;
+    while (t >= tiles_c - by) {
+      t -= tiles_c - by;
+      ++by;
+    }
+    bx = by + t;
+  } else {
+    bx = blockIdx.x % tiles_c;
+    by = blockIdx.x / tiles_c;
+  }
   const int R2 = 2 * nrows, C2 = 2 * ncols;
-  const int r0 = blockIdx.y * (kTM / 2), c0 = blockIdx.x * (kTN / 2);
+  const int r0 = by * (kTM / 2), c0 = bx * (kTN / 2);
   const int x0 = blockIdx.z * kchunk;
   const int x1 = min(n3, x0 + kchunk);
-  double acc[4][4];
+  d4 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+  // each thread stages two (grid point, pair) elements of A and of B per stage
+  int2 rab[2];
+  const z64* bcol[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int rr = (tid + 256 * q) / kKC;
+    rab[q] = (r0 + rr < nrows) ? rows[r0 + rr] : int2{-1, -1};
+    bcol[q] = (c0 + rr < ncols) ? tr + (size_t)cols[c0 + rr] * n3 : nullptr;
+  }
   for (int xb = x0; xb < x1; xb += kKC) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -54,8 +78,12 @@ __global__ void __launch_bounds__(256) k_gemm_tn(const z64* __restrict__ D, cons
       const int x = xb + xx;
       z64 va = mk(0.0, 0.0), vb = mk(0.0, 0.0);
       if (x < x1) {
-        if (r0 + rr < nrows) va = D[(size_t)(r0 + rr) * n3 + x];
-        if (c0 + rr < ncols) vb = tr[(size_t)cols[c0 + rr] * n3 + x];
+        if (rab[q].x >= 0) {  // rho = phi1(:,a) * CONJUGATE(phi2(:,b))   SD/coreproject.f90:1094
+          z64 w = phi2[(size_t)rab[q].y * n3 + x];
+          if (conj_b) w.im = -w.im;
+          va = phi1[(size_t)rab[q].x * n3 + x] * w;
+        }
+        if (bcol[q]) vb = bcol[q][x];
       }
       As[xx][2 * rr] = va.re;
       As[xx][2 * rr + 1] = va.im;
@@ -64,37 +92,42 @@ __global__ void __launch_bounds__(256) k_gemm_tn(const z64* __restrict__ D, cons
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < kKC; ++kk) {
-      double a[4], b[4];
+    for (int kk = 0; kk < kKC; kk += 4) {
+      double a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+      for (int i = 0; i < 2; ++i) a[i] = As[kk + (lane >> 4)][wr * 32 + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+      for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wc * 32 + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
   double* p = part + (size_t)blockIdx.z * R2 * C2;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int row = 2 * r0 + ty * 4 + i;
-    if (row >= R2) continue;
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int col = 2 * c0 + tx * 4 + j;
-      if (col < C2) p[(size_t)row * C2 + col] = acc[i][j];
-    }
-  }
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 2 * r0 + wr * 32 + i * 16 + 4 * v + (lane >> 4);
+        const int col = 2 * c0 + wc * 32 + j * 16 + (lane & 15);
+        if (row < R2 && col < C2) p[(size_t)row * C2 + col] = acc[i][j][v];
+      }
 }
 
-__global__ void k_reduce_parts(const double* __restrict__ part, int nsplit, long len, double* __restrict__ out) {
+__global__ void k_reduce_parts(const double* __restrict__ part, int nsplit, int R2, int C2, int sym,
+                               double* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long len = (long)R2 * C2;
   if (i >= len) return;
+  const int row = (int)(i / C2), col = (int)(i % C2);
+  // with the symmetric tiling only tiles with col-tile >= row-tile exist
+  const long src = (sym && col / kTN < row / kTM) ? (long)col * C2 + row : i;
   double s = 0.0;
-  for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * len + i];
+  for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * len + src];
   out[i] = s;
 }
 
@@ -103,34 +136,49 @@ static int ksplit_for(int tiles) {
   return ks < 1 ? 1 : (ks > 256 ? 256 : ks);
 }
 
-size_t matel_scratch_bytes(int n3, int nrows, int ncols) {
-  const int tiles = ((2 * nrows + kTM - 1) / kTM) * ((2 * ncols + kTN - 1) / kTN);
-  const int ks = ksplit_for(tiles);
-  return (size_t)nrows * n3 * sizeof(z64) + (size_t)ks * (2 * nrows) * (2 * ncols) * sizeof(double) + 256;
+// Tiling of one reduction: output tiles (upper triangle only when symmetric)
+// and the split of the grid-point sum.  Shared by the scratch sizing and the
+// launch so the two can never disagree.
+struct MatelTiling {
+  int tilesR, tilesC, ntiles, kchunk, ksplit;
+  bool sym;
+};
+static MatelTiling matel_tiling(int n3, int nrows, int ncols, bool symmetric) {
+  MatelTiling t;
+  t.tilesR = (2 * nrows + kTM - 1) / kTM;
+  t.tilesC = (2 * ncols + kTN - 1) / kTN;
+  t.sym = symmetric && nrows == ncols;
+  t.ntiles = t.sym ? t.tilesC * (t.tilesC + 1) / 2 : t.tilesR * t.tilesC;
+  const int ks = ksplit_for(t.ntiles);
+  t.kchunk = (n3 + ks - 1) / ks;
+  t.kchunk = ((t.kchunk + kKC - 1) / kKC) * kKC;
+  t.ksplit = (n3 + t.kchunk - 1) / t.kchunk;
+  return t;
+}
+
+size_t matel_scratch_bytes(int n3, int nrows, int ncols, int symmetric) {
+  const MatelTiling t = matel_tiling(n3, nrows, ncols, symmetric != 0);
+  return (size_t)t.ksplit * (2 * nrows) * (2 * ncols) * sizeof(double) + 256;
 }
 
 int matel_launch(const MatelArgsHost& a, hipStream_t st, Prof* prof) {
   if (a.nrows <= 0 || a.ncols <= 0) return SDV_OK;
-  const int tilesR = (2 * a.nrows + kTM - 1) / kTM, tilesC = (2 * a.ncols + kTN - 1) / kTN;
-  const int ks = ksplit_for(tilesR * tilesC);
-  int kchunk = (a.n3 + ks - 1) / ks;
-  kchunk = ((kchunk + kKC - 1) / kKC) * kKC;
-  const int ksplit = (a.n3 + kchunk - 1) / kchunk;
-  z64* D = (z64*)a.scratch;
-  double* part = (double*)((char*)a.scratch + (size_t)a.nrows * a.n3 * sizeof(z64));
+  const MatelTiling t = matel_tiling(a.n3, a.nrows, a.ncols, a.symmetric != 0);
+  const bool sym = t.sym;
+  const int tilesC = t.tilesC, ntiles = t.ntiles, kchunk = t.kchunk, ksplit = t.ksplit;
+  SDV_REQUIRE(a.scratch_bytes >= matel_scratch_bytes(a.n3, a.nrows, a.ncols, a.symmetric), SDV_ERR_ARG,
+              "matrix-element scratch buffer too small");
+  double* part = (double*)a.scratch;
   hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL(k_form_rho, dim3(256, a.nrows), dim3(256), 0, st, (const z64*)a.phi1, (const z64*)a.phi2,
-                     a.rows_dev, a.n3, a.conj_b, D);
-  SDV_HIP_CHECK(hipGetLastError());
-  if (prof) prof->end(SDV_STAGE_MATEL_RHO, e0, st);
-  e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL(k_gemm_tn, dim3(tilesC, tilesR, ksplit), dim3(256), 0, st, D, (const z64*)a.tr, a.cols_dev,
-                     a.nrows, a.ncols, a.n3, kchunk, part);
+  hipLaunchKernelGGL(k_gemm_mfma, dim3(ntiles, 1, ksplit), dim3(256), 0, st, (const z64*)a.phi1, (const z64*)a.phi2,
+                     a.rows_dev, a.conj_b, (const z64*)a.tr, a.cols_dev, a.nrows, a.ncols, a.n3, kchunk,
+                     sym ? 1 : 0, tilesC, part);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_MATEL_GEMM, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
   const long len = (long)(2 * a.nrows) * (2 * a.ncols);
-  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, ksplit, len, a.gout);
+  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, st, part, ksplit,
+                     2 * a.nrows, 2 * a.ncols, sym ? 1 : 0, a.gout);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_MATEL_REDUCE, e0, st);
   return SDV_OK;

@@ -23,7 +23,7 @@ EXPORTS = [
     "sdv_twoereduced", "sdv_twoereduced_dev", "sdv_profile", "sdv_profile_read",
     "sdv_get_kernel_octant", "sdv_get_ke_stencil", "sdv_circulant_edge",
 ]
-STAGES = ["xfwd", "yz", "xinv", "matel_rho", "matel_gemm", "matel_reduce", "ke"]
+STAGES = ["xfwd", "yz", "xinv", "matel_assemble", "matel_gemm", "matel_reduce", "ke"]
 
 
 class SdvConfig(ctypes.Structure):
