@@ -64,6 +64,14 @@ int sdv_free(sdv_ctx* ctx);
 const char* sdv_last_error(void);
 int sdv_set_precision(sdv_ctx* ctx, int precision);
 
+/* Nuclear attraction potential of get_3dpoisson / get_twoe_new
+ * (SD/sincproject.f90:192-250,311-338): pot(totpoints) real, for numcenters
+ * nuclei at centershift(3, numcenters) half-spacing offsets with charges
+ * nuccharges(numcenters).  realkind4 = 1 evaluates the far-field sqrt in
+ * 4-byte REAL as builds without -r8 do (:220); 0 in 8-byte REAL. */
+int sdv_nuclear_potential(sdv_ctx* ctx, int numcenters, const int* centershift, const double* nuccharges,
+                          int realkind4, double* pot);
+
 /* op_tinv (module tinvsubmod, SD/coreproject.f90:250-258):
  *   twoeden03(totpoints, allsize) in, twoereduced(totpoints, allsize) out.
  *   allsize must be a multiple of circsize (:387-391). */

@@ -230,3 +230,16 @@ def reducedpot(twoereduced, twoden):
     out = np.zeros((n3, N, N), dtype=np.complex128, order="F")
     lib().orc_reducedpot(ctypes.c_int(n3), ctypes.c_int(N), _p(tr), _p(g), _p(out))
     return out
+
+
+def nuclear_pot(n, spacing, kernel, centershift, charges, realkind4=False):
+    """get_3dpoisson's nuclear potential on the grid, (n^3,) reals."""
+    cs = np.ascontiguousarray(np.asarray(centershift, dtype=np.int32).reshape(-1, 3))
+    Z = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+    pot = np.zeros(n ** 3)
+    rc = lib().orc_nuclear_pot(ctypes.c_int(n), ctypes.c_double(spacing), _p(np.asfortranarray(kernel)),
+                               ctypes.c_int(len(Z)), cs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(Z),
+                               ctypes.c_int(int(realkind4)), _p(pot))
+    if rc:
+        raise ValueError("centershift parity error (SD/sincproject.f90:199-204)")
+    return pot

@@ -693,3 +693,43 @@ void orc_reducedpot(int n3_, int N, const double *tr_, const double *twoden_, do
       out[x + n3 * c] = acc;
     }
 }
+
+/* a4 (nuclear part): get_3dpoisson's potential  -- SD/sincproject.f90:192-250.
+ * kernel: (2n-1)^3 threed_two values (offsets -(n-1)..(n-1)).  Each centre
+ * sits at half-spacing offsets centershift (3 ints); grid point i (0-based)
+ * sees offset o = i + (cs - n + 1)/2.  Inside the cube |o| <= n-1 the value
+ * is the kernel (so the nucleus sits at -cs*spacing/2); outside it is
+ * 1/(spacing*sqrt(o.o)) with the sqrt taken in
+ * default REAL (realkind4 = 1: 4-byte, builds without -r8; 0: 8-byte).
+ * pot = - sum_c Z_c value_c.  Returns -1 on the parity error of :199-204.   */
+int orc_nuclear_pot(int n, double spacing, const double *kernel, int ncent, const int *cs, const double *Z,
+                    int realkind4, double *pot) {
+  const int L = 2 * n - 1;
+  for (int c = 0; c < ncent; c++)
+    for (int d = 0; d < 3; d++) {
+      int pp = -n + cs[3 * c + d] + 1, qq = n + cs[3 * c + d] - 1;
+      if ((pp % 2) != 0 || (qq % 2) != 0) return -1;
+    }
+  const size_t n3 = (size_t)n * n * n;
+  for (size_t p = 0; p < n3; p++) pot[p] = 0.0;
+  for (int c = 0; c < ncent; c++) {
+    int off[3];
+    for (int d = 0; d < 3; d++) off[d] = (-n + cs[3 * c + d] + 1) / 2;
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+          int ox = i + off[0], oy = j + off[1], oz = k + off[2];
+          double v;
+          if (abs(ox) <= n - 1 && abs(oy) <= n - 1 && abs(oz) <= n - 1) {
+            v = kernel[(ox + n - 1) + (size_t)L * ((oy + n - 1) + (size_t)L * (oz + n - 1))];
+          } else {
+            int s2 = ox * ox + oy * oy + oz * oz;
+            double rval = realkind4 ? spacing * (double)sqrtf((float)s2) : spacing * sqrt((double)s2);
+            v = 1.0 / rval;
+          }
+          size_t p = i + (size_t)n * (j + (size_t)n * k);
+          pot[p] = pot[p] - Z[c] * v;                                  /* :245-247 */
+        }
+  }
+  return 0;
+}

@@ -206,3 +206,18 @@ def test_twoe_matel_16_orbitals_small_grid(orc, torch_dev, lowhigh):
     otm, otr = orc.twoe_matel(n, kern, phi, phi, low=low, high=high)
     assert relerr(tr, otr) < TOL64
     assert relerr(tm, otm) < TOL64
+
+
+@pytest.mark.parametrize("realkind4", [False, True])
+def test_nuclear_potential(orc, torch_dev, realkind4):
+    """Two nuclei of charge 1 at +-0.6 bohr on z (centershift +-4) on the
+    41^3, 0.3-bohr grid, and one charge-2 nucleus at the origin on 21^3."""
+    from sincdvr import SdvError, SincDVR
+    for n, dx, cs, zc in [(41, 0.3, [[0, 0, 4], [0, 0, -4]], [1.0, 1.0]), (21, 0.4, [[0, 0, 0]], [2.0])]:
+        say("nuclear potential n=%d realkind4=%s" % (n, realkind4))
+        s = SincDVR(n, dx, 1)
+        kern = orc.coulomb_kernel(n, dx)
+        ref = orc.nuclear_pot(n, dx, kern, cs, zc, realkind4=realkind4)
+        assert relerr(s.nuclear_potential(cs, zc, realkind4=realkind4), ref) < 1e-13
+    with pytest.raises(SdvError):
+        s.nuclear_potential([[1, 0, 0]], [1.0])

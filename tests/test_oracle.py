@@ -279,3 +279,39 @@ def test_mean_field(orc):
     out = orc.mult_reducedpot(2, 3, phi, rp[:, :, 1:3])
     ref2 = np.einsum("xki,xk->xi", rp[:, :, 1:3], phi)
     assert relerr(out, ref2) < 1e-14
+
+
+# ------------------------------------------------- a4 nuclear potential
+def test_nuclear_potential_oracle(orc):
+    """SD/sincproject.f90:192-250.  Grid point i of a nucleus with half-spacing
+    shift cs sees offset i + (cs - n + 1)/2, i.e. the nucleus sits at
+    -cs*spacing/2.  Outside the kernel cube pot = -Z/r exactly (8-byte sqrt),
+    inside it is -Z times the corrected kernel; the 4-byte sqrt of builds
+    without -r8 differs only outside; the centershift parity rule holds."""
+    n, d = 9, 0.4
+    kern = orc.coulomb_kernel(n, d, nbig=40, nsmall=8)
+    pts = (np.arange(1, n + 1) * d) - (d + n * d) / 2
+    X, Y, Z = np.meshgrid(pts, pts, pts, indexing="ij")
+    cs, zc = [[0, 0, 16], [0, 0, -4]], [1.0, 2.0]
+    pot = orc.nuclear_pot(n, d, kern, cs, zc).reshape((n, n, n), order="F")
+    ref = np.zeros_like(pot)
+    inside_all = np.ones((n, n, n), bool)
+    for (cx, cy, cz), q in zip(cs, zc):
+        o = [np.arange(n) + (c - n + 1) // 2 for c in (cx, cy, cz)]
+        O = np.meshgrid(*o, indexing="ij")
+        inside = (abs(O[0]) <= n - 1) & (abs(O[1]) <= n - 1) & (abs(O[2]) <= n - 1)
+        inside_all &= inside
+        far = -q / (d * np.sqrt(np.where(inside, 1, O[0] ** 2 + O[1] ** 2 + O[2] ** 2)))
+        kv = np.zeros_like(far)
+        kv[inside] = -q * kern[(O[0] + n - 1)[inside], (O[1] + n - 1)[inside], (O[2] + n - 1)[inside]]
+        ref += np.where(inside, kv, far)
+        # the nucleus is at -cs*spacing/2: far values are -Z/|r - r_c|
+        r = np.sqrt((X + cx * d / 2) ** 2 + (Y + cy * d / 2) ** 2 + (Z + cz * d / 2) ** 2)
+        assert np.allclose(far[~inside], -q / r[~inside], rtol=1e-14)
+    assert np.allclose(pot, ref, rtol=1e-14, atol=0)
+    assert (~inside_all).any() and inside_all.any()
+    p4 = orc.nuclear_pot(n, d, kern, cs, zc, realkind4=True).reshape((n, n, n), order="F")
+    assert np.array_equal(p4[inside_all], pot[inside_all])
+    assert 0 < np.abs(p4 - pot).max() < 1e-6 * np.abs(pot).max()
+    with pytest.raises(ValueError):
+        orc.nuclear_pot(n, d, kern, [[1, 0, 0]], [1.0])                # odd shift with odd n

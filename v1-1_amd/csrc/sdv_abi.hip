@@ -266,6 +266,42 @@ int sdv_set_precision(sdv_ctx* c, int p) {
 
 int sdv_circulant_edge(sdv_ctx* c) { return c ? c->spec.g.M : SDV_ERR_ARG; }
 
+int sdv_nuclear_potential(sdv_ctx* c, int numcenters, const int* centershift, const double* nuccharges,
+                          int realkind4, double* pot) {
+  SDV_REQUIRE(c && pot && numcenters >= 0, SDV_ERR_ARG, "bad argument");
+  SDV_REQUIRE(numcenters == 0 || (centershift && nuccharges), SDV_ERR_ARG, "null centre arrays");
+  const int n = c->n;
+  for (int i = 0; i < 3 * numcenters; ++i) {  // SD/sincproject.f90:196-204
+    const int pp = -n + centershift[i] + 1, qq = n + centershift[i] - 1;
+    SDV_REQUIRE(pp % 2 == 0 && qq % 2 == 0, SDV_ERR_ARG,
+                "centershift parity: with numpoints odd centershift must be even and vice versa");
+  }
+  int* dcs = nullptr;
+  double *dz = nullptr, *dpot = nullptr;
+  const size_t n3 = (size_t)c->n3;
+  int rc = SDV_OK;
+  if (hipMalloc(&dcs, 3 * (numcenters + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc(&dz, (numcenters + 1) * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dpot, n3 * sizeof(double)) != hipSuccess) {
+    set_error("device allocation failed in sdv_nuclear_potential");
+    rc = SDV_ERR_NOMEM;
+  }
+  if (rc == SDV_OK && numcenters > 0) {
+    if (hipMemcpy(dcs, centershift, 3 * numcenters * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dz, nuccharges, numcenters * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      rc = SDV_ERR_HIP;
+  }
+  if (rc == SDV_OK) rc = nuclear_pot_launch(c->spec, c->cfg.spacing, numcenters, dcs, dz, realkind4, dpot, c->stream);
+  if (rc == SDV_OK && (hipMemcpyAsync(pot, dpot, n3 * sizeof(double), hipMemcpyDefault, c->stream) != hipSuccess ||
+                       hipStreamSynchronize(c->stream) != hipSuccess))
+    rc = SDV_ERR_HIP;
+  hipFree(dcs);
+  hipFree(dz);
+  hipFree(dpot);
+  if (rc == SDV_ERR_HIP) set_error("HIP error in sdv_nuclear_potential");
+  return rc;
+}
+
 int sdv_get_kernel_octant(sdv_ctx* c, double* koct) {
   SDV_REQUIRE(c && koct, SDV_ERR_ARG, "null argument");
   SDV_HIP_CHECK(hipMemcpy(koct, c->spec.koct, (size_t)c->n3 * sizeof(double), hipMemcpyDeviceToHost));

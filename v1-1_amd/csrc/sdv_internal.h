@@ -77,6 +77,8 @@ size_t conv_work_bytes(const Geom& g, int npairs, int precision);
 // Kernel construction (tooth + placement + spectrum), FP64.
 int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st);
 void free_spectrum(Spectrum* s);
+int nuclear_pot_launch(const Spectrum& s, double spacing, int ncent, const int* cs_dev, const double* z_dev,
+                       int realkind4, double* pot_dev, hipStream_t st);
 
 // twoematel reduction: G = sum_x D_row(x) V_col(x)
 struct MatelArgsHost {

@@ -213,6 +213,45 @@ __global__ void k_twiddles(z64* tw64, c32* tw32, int M) {
   tw32[k] = mk((float)c, (float)-s);
 }
 
+// Nuclear potential of get_3dpoisson (SD/sincproject.f90:192-250): a centre
+// at half-spacing offsets cs sees grid point i at offset o = i + (cs-n+1)/2;
+// inside the cube |o| <= n-1 the value is the kernel threed_two (from the
+// octant, the kernel being even in each axis), outside it is
+// 1/(spacing*sqrt(o.o)) with the sqrt in default REAL (realkind4: 4-byte,
+// builds without -r8).  pot = sum over centres of -Z_c * value.
+__global__ void k_nuclear_pot(int n, double spacing, const double* koct, int ncent, const int* cs, const double* Z,
+                              int realkind4, double* pot) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * n * n) return;
+  const int i = idx % n, j = (idx / n) % n, k = idx / ((long)n * n);
+  double acc = 0.0;
+  for (int c = 0; c < ncent; ++c) {
+    const int ox = i + (-n + cs[3 * c] + 1) / 2;
+    const int oy = j + (-n + cs[3 * c + 1] + 1) / 2;
+    const int oz = k + (-n + cs[3 * c + 2] + 1) / 2;
+    double v;
+    if (abs(ox) <= n - 1 && abs(oy) <= n - 1 && abs(oz) <= n - 1) {
+      v = koct[abs(ox) + (long)n * (abs(oy) + (long)n * abs(oz))];
+    } else {
+      const int s2 = ox * ox + oy * oy + oz * oz;
+      const double rval = realkind4 ? spacing * (double)sqrtf((float)s2) : spacing * sqrt((double)s2);
+      v = 1.0 / rval;
+    }
+    acc = acc - Z[c] * v;
+  }
+  pot[idx] = acc;
+}
+
+int nuclear_pot_launch(const Spectrum& s, double spacing, int ncent, const int* cs_dev, const double* z_dev,
+                       int realkind4, double* pot_dev, hipStream_t st) {
+  const int n = s.g.n;
+  const long n3 = (long)n * n * n;
+  hipLaunchKernelGGL(k_nuclear_pot, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, n, spacing, s.koct, ncent,
+                     cs_dev, z_dev, realkind4, pot_dev);
+  SDV_HIP_CHECK(hipGetLastError());
+  return SDV_OK;
+}
+
 static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st) {

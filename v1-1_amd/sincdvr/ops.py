@@ -102,6 +102,17 @@ class SincDVR:
         _lib.check(self._lib.sdv_get_kernel_octant(self._h, ctypes.c_void_p(out.ctypes.data)))
         return out
 
+    def nuclear_potential(self, centershift, charges, realkind4=False):
+        """SD/sincproject.f90:192-250: pot(totpoints) for nuclei at half-spacing
+        offsets centershift (ncentres, 3) with the given charges."""
+        cs = np.ascontiguousarray(np.asarray(centershift, dtype=np.int32).reshape(-1, 3))
+        z = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        pot = np.zeros(self.totpoints)
+        _lib.check(self._lib.sdv_nuclear_potential(self._h, len(z), ctypes.c_void_p(cs.ctypes.data),
+                                                   ctypes.c_void_p(z.ctypes.data), int(realkind4),
+                                                   ctypes.c_void_p(pot.ctypes.data)))
+        return pot
+
     def ke_stencil(self):
         out = np.zeros(2 * self.n - 1)
         _lib.check(self._lib.sdv_get_ke_stencil(self._h, ctypes.c_void_p(out.ctypes.data)))
