@@ -116,6 +116,36 @@ int sdv_twoereduced(sdv_ctx* ctx, int lowspf, int highspf, const double* inspfs1
 int sdv_twoereduced_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs10, const void* inspfs20,
                         void* twoereduced, void* stream);
 
+/* Batched form: the pair potentials of many independent orbital sets, of
+ * uneven grid sizes and orbital counts, in one persistent launch per pipeline
+ * stage (three launches per step).  Each set is a context (its grid, kernel
+ * spectrum, numspf) with device arrays inspfs(totpoints, numspf) and
+ * twoereduced(totpoints, numspf, numspf); the pair-symmetric path of
+ * call_twoe_matel00 (one array as bra and ket, lowspf..highspf = 1..numspf)
+ * is computed for every set.  No counterpart in the reference (which runs
+ * one orbital set per process); the per-set result equals that of
+ * sdv_twoereduced_dev on the same set.
+ * sdv_batch_capture records one step into a hipGraph on `stream`; later
+ * sdv_batch_run calls on that stream replay it.  sdv_batch_set_grid sets the
+ * persistent grid to grid_mult x (compute units) workgroups (autotuning). */
+typedef struct sdv_batch sdv_batch;
+typedef struct sdv_batch_set {
+  sdv_ctx* ctx;
+  int numspf;
+  const void* inspfs;
+  void* twoereduced;
+} sdv_batch_set;
+int sdv_batch_create(int nsets, const sdv_batch_set* sets, int precision, sdv_batch** batch);
+int sdv_batch_set_grid(sdv_batch* batch, int grid_mult);
+int sdv_batch_run(sdv_batch* batch, void* stream);
+int sdv_batch_capture(sdv_batch* batch, void* stream);
+int sdv_batch_stats(sdv_batch* batch, long long* npairs, long long* work_bytes);
+int sdv_batch_free(sdv_batch* batch);
+/* Per-stage timing of sdv_batch_run (stages XFWD, YZ, XINV), as sdv_profile /
+ * sdv_profile_read below; graph replays are not timed per stage. */
+int sdv_batch_profile(sdv_batch* batch, int enable);
+int sdv_batch_profile_read(sdv_batch* batch, double* ms, int* count);
+
 /* Per-kernel timing with HIP events recorded on the launch stream, the
  * library's counterpart of the reference's twoematel.time.dat / zke.time.dat
  * counters (SD/coreproject.f90:1114-1142,1499-1517).  sdv_profile(ctx, 1)

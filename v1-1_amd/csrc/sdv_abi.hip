@@ -521,25 +521,26 @@ int sdv_call_twoe_matel00_dev(sdv_ctx* c, int low, int high, const void* phi1, c
   return SDV_OK;
 }
 
-int sdv_profile(sdv_ctx* c, int enable) {
-  SDV_REQUIRE(c, SDV_ERR_ARG, "null context");
+// ---- multi-set batch
+}  // extern "C"
+
+static int prof_reset(Prof* p, int enable) {
   SDV_HIP_CHECK(hipDeviceSynchronize());
-  for (auto& r : c->prof.recs) {
+  for (auto& r : p->recs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
-  c->prof.recs.clear();
-  c->prof.on = enable != 0;
+  p->recs.clear();
+  p->on = enable != 0;
   return SDV_OK;
 }
 
-int sdv_profile_read(sdv_ctx* c, double* ms, int* count) {
-  SDV_REQUIRE(c && ms && count, SDV_ERR_ARG, "null argument");
+static int prof_read(const Prof& p, double* ms, int* count) {
   for (int i = 0; i < SDV_NSTAGES; ++i) {
     ms[i] = 0.0;
     count[i] = 0;
   }
-  for (auto& r : c->prof.recs) {
+  for (const auto& r : p.recs) {
     SDV_HIP_CHECK(hipEventSynchronize(r.b));
     float t = 0.f;
     SDV_HIP_CHECK(hipEventElapsedTime(&t, r.a, r.b));
@@ -549,6 +550,185 @@ int sdv_profile_read(sdv_ctx* c, double* ms, int* count) {
     }
   }
   return SDV_OK;
+}
+
+struct sdv_batch {
+  std::vector<sdv_ctx*> ctxs;
+  std::vector<BatchSetHost> sets;
+  BatchPlan plan;
+  int precision = 0;
+  int grid_mult = 1;
+  void* dsets = nullptr;
+  void* ditems = nullptr;
+  void* dwx = nullptr;
+  void* dwy = nullptr;
+  void* X = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t graph_stream = nullptr;
+  Prof prof;
+};
+
+static void batch_release(sdv_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  (void)hipFree(b->dsets);
+  (void)hipFree(b->ditems);
+  (void)hipFree(b->dwx);
+  (void)hipFree(b->dwy);
+  (void)hipFree(b->X);
+  b->dsets = b->ditems = b->dwx = b->dwy = b->X = nullptr;
+}
+
+static int batch_build(sdv_batch* b) {
+  batch_release(b);
+  int dev = 0, ncu = 256;
+  SDV_HIP_CHECK(hipGetDevice(&dev));
+  SDV_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  b->plan = batch_plan(b->sets, b->precision, b->grid_mult, ncu);
+  SDV_REQUIRE(b->plan.grid_x > 0 && b->plan.grid_y > 0, SDV_ERR_UNSUPPORTED, "batch holds an unsupported grid");
+  const size_t sb = batch_set_bytes(b->precision);
+  std::vector<char> hs(sb * b->sets.size());
+  for (size_t i = 0; i < b->sets.size(); ++i) batch_fill_set(b->precision, b->sets[i], hs.data() + i * sb);
+  const BatchPlan& p = b->plan;
+  if (hipMalloc(&b->dsets, hs.size() + 16) != hipSuccess ||
+      hipMalloc(&b->ditems, p.items.size() * batch_item_bytes() + 16) != hipSuccess ||
+      hipMalloc(&b->dwx, p.wx.size() * batch_work_bytes() + 16) != hipSuccess ||
+      hipMalloc(&b->dwy, p.wy.size() * batch_work_bytes() + 16) != hipSuccess ||
+      hipMalloc(&b->X, p.x_bytes + 256) != hipSuccess) {
+    set_error("device allocation failed for the batch");
+    batch_release(b);
+    return SDV_ERR_NOMEM;
+  }
+  SDV_HIP_CHECK(hipMemcpy(b->dsets, hs.data(), hs.size(), hipMemcpyHostToDevice));
+  SDV_HIP_CHECK(hipMemcpy(b->ditems, p.items.data(), p.items.size() * batch_item_bytes(), hipMemcpyHostToDevice));
+  SDV_HIP_CHECK(hipMemcpy(b->dwx, p.wx.data(), p.wx.size() * batch_work_bytes(), hipMemcpyHostToDevice));
+  SDV_HIP_CHECK(hipMemcpy(b->dwy, p.wy.data(), p.wy.size() * batch_work_bytes(), hipMemcpyHostToDevice));
+  return SDV_OK;
+}
+
+static BatchDev batch_dev(const sdv_batch* b) {
+  BatchDev d;
+  d.sets = b->dsets;
+  d.items = (const BItemDev*)b->ditems;
+  d.wx = (const BWork*)b->dwx;
+  d.wy = (const BWork*)b->dwy;
+  d.X = b->X;
+  return d;
+}
+
+extern "C" {
+
+int sdv_batch_create(int nsets, const sdv_batch_set* sets, int precision, sdv_batch** out) {
+  SDV_REQUIRE(out && sets && nsets > 0, SDV_ERR_ARG, "bad batch arguments");
+  SDV_REQUIRE(precision == 0 || precision == 1, SDV_ERR_ARG, "precision must be 0 or 1");
+  sdv_batch* b = new sdv_batch();
+  b->precision = precision;
+  for (int i = 0; i < nsets; ++i) {
+    sdv_ctx* c = sets[i].ctx;
+    if (!c || !sets[i].inspfs || !sets[i].twoereduced || sets[i].numspf != c->N) {
+      set_error("batch set " + std::to_string(i) + ": null pointer or numspf differs from its context");
+      delete b;
+      return SDV_ERR_ARG;
+    }
+    IdxPlan* pl = nullptr;
+    int rc = twoe_plan(c, 1, c->N, true, &pl);
+    if (rc != SDV_OK) {
+      delete b;
+      return rc;
+    }
+    BatchSetHost h;
+    h.spec = &c->spec;
+    h.items = &pl->items;
+    h.phi = sets[i].inspfs;
+    h.out = sets[i].twoereduced;
+    h.cnorm = c->cfg.cnormflag;
+    b->sets.push_back(h);
+    b->ctxs.push_back(c);
+  }
+  int rc = batch_build(b);
+  if (rc != SDV_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return SDV_OK;
+}
+
+int sdv_batch_set_grid(sdv_batch* b, int grid_mult) {
+  SDV_REQUIRE(b && grid_mult >= 1 && grid_mult <= 64, SDV_ERR_ARG, "grid multiplier must be in [1, 64]");
+  b->grid_mult = grid_mult;
+  return batch_build(b);
+}
+
+int sdv_batch_run(sdv_batch* b, void* stream) {
+  SDV_REQUIRE(b, SDV_ERR_ARG, "null batch");
+  hipStream_t st = (hipStream_t)stream;
+  if (b->exec && st == b->graph_stream) {
+    SDV_HIP_CHECK(hipGraphLaunch(b->exec, st));
+    return SDV_OK;
+  }
+  return batch_launch(b->plan, batch_dev(b), st, &b->prof);
+}
+
+int sdv_batch_capture(sdv_batch* b, void* stream) {
+  SDV_REQUIRE(b && stream, SDV_ERR_ARG, "capture needs a batch and a non-default stream");
+  hipStream_t st = (hipStream_t)stream;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  SDV_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_launch(b->plan, batch_dev(b), st, nullptr);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc != SDV_OK) return rc;
+  SDV_HIP_CHECK(e);
+  b->graph = g;
+  SDV_HIP_CHECK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_stream = st;
+  return SDV_OK;
+}
+
+int sdv_batch_stats(sdv_batch* b, long long* npairs, long long* work_bytes) {
+  SDV_REQUIRE(b, SDV_ERR_ARG, "null batch");
+  if (npairs) *npairs = (long long)b->plan.items.size();
+  if (work_bytes) *work_bytes = (long long)b->plan.x_bytes;
+  return SDV_OK;
+}
+
+int sdv_batch_free(sdv_batch* b) {
+  if (!b) return SDV_OK;
+  (void)hipDeviceSynchronize();
+  batch_release(b);
+  for (auto& r : b->prof.recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  delete b;
+  return SDV_OK;
+}
+
+int sdv_profile(sdv_ctx* c, int enable) {
+  SDV_REQUIRE(c, SDV_ERR_ARG, "null context");
+  return prof_reset(&c->prof, enable);
+}
+
+int sdv_profile_read(sdv_ctx* c, double* ms, int* count) {
+  SDV_REQUIRE(c && ms && count, SDV_ERR_ARG, "null argument");
+  return prof_read(c->prof, ms, count);
+}
+
+int sdv_batch_profile(sdv_batch* b, int enable) {
+  SDV_REQUIRE(b, SDV_ERR_ARG, "null batch");
+  return prof_reset(&b->prof, enable);
+}
+
+int sdv_batch_profile_read(sdv_batch* b, double* ms, int* count) {
+  SDV_REQUIRE(b && ms && count, SDV_ERR_ARG, "null argument");
+  return prof_read(b->prof, ms, count);
 }
 
 int sdv_mult_ke_dev(sdv_ctx* c, const void* in, void* out, int howmany, void* stream) {

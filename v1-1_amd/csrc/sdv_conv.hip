@@ -21,6 +21,9 @@
 // Each m-point line FFT is a two-step Cooley-Tukey (P-point DFT in registers,
 // twiddle, LDS exchange, Q-point DFT in registers); frequency k1 + P k2 is
 // kept at position k1 Q + k2 and the inverse consumes that order directly.
+#include <algorithm>
+#include <cstring>
+
 #include "sdv_dft.h"
 #include "sdv_internal.h"
 
@@ -65,6 +68,46 @@ struct KArgs {
   int lds_tw2, lds_y, lds_z2, lds_in;  // runtime LDS offsets (elements) of the k_yz tables and workspaces
 };
 
+// Everything one work item (one pair's convolution) needs, resolved from the
+// single-set arguments or from a multi-set batch table.
+template <typename T>
+struct ItemView {
+  int n, n2, n3;
+  const z64* pa;       // phi1 column a (orbital mode) or density column (density mode)
+  const z64* pb;       // phi2 column b
+  int density_mode, conj_b, mirror_conj;
+  cplx<T>* X;          // this pair's 2m planes of n^2
+  z64* out;            // output column
+  z64* outm;           // mirror output column or nullptr
+  const cplx<T>* tw;
+  const T* khat;
+  int lds_tw2, lds_y, lds_z2, lds_in;
+};
+
+template <typename T>
+__device__ __forceinline__ ItemView<T> make_view(const KArgs<T>& a, int pidx, int R) {
+  const PairItem it = a.pairs[pidx];
+  ItemView<T> v;
+  v.n = a.n;
+  v.n2 = a.n2;
+  v.n3 = a.n3;
+  v.pa = a.phi1 + (size_t)it.a * a.n3;
+  v.pb = a.phi2 + (size_t)it.b * a.n3;
+  v.density_mode = a.density_mode;
+  v.conj_b = a.conj_b;
+  v.mirror_conj = a.mirror_conj;
+  v.X = a.X + (size_t)pidx * 2 * R * a.n2;
+  v.out = a.out + (size_t)it.col * a.n3;
+  v.outm = it.mcol >= 0 ? a.out + (size_t)it.mcol * a.n3 : nullptr;
+  v.tw = a.tw;
+  v.khat = a.khat;
+  v.lds_tw2 = a.lds_tw2;
+  v.lds_y = a.lds_y;
+  v.lds_z2 = a.lds_z2;
+  v.lds_in = a.lds_in;
+  return v;
+}
+
 template <int P, int Q>
 struct XCfg {
   static constexpr int MX = P > Q ? P : Q;
@@ -81,7 +124,7 @@ struct YZCfg {
 
 // ---------------------------------------------------------------- x forward
 template <typename T, int P, int Q>
-__device__ __forceinline__ void xfwd_step2(const KArgs<T>& a, const cplx<T>* buf, int pidx, int l0, int nl, int c) {
+__device__ __forceinline__ void xfwd_step2(const ItemView<T>& a, const cplx<T>* buf, int l0, int nl, int c) {
   constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1;
   const int tid = threadIdx.x;
   const int line = tid % LA, k1 = tid / LA;
@@ -90,27 +133,25 @@ __device__ __forceinline__ void xfwd_step2(const KArgs<T>& a, const cplx<T>* buf
 #pragma unroll
     for (int j2 = 0; j2 < Q; ++j2) u[j2] = buf[line * pitch + k1 * Q + j2];
     DFT<T, Q, false>::run(u);
-    cplx<T>* dst = a.X + ((size_t)pidx * 2 * R + (size_t)c * R + (size_t)k1 * Q) * a.n2 + l0 + line;
+    cplx<T>* dst = a.X + ((size_t)c * R + (size_t)k1 * Q) * a.n2 + l0 + line;
 #pragma unroll
     for (int k2 = 0; k2 < Q; ++k2) dst[(size_t)k2 * a.n2] = u[k2];
   }
 }
 
+// x forward of lines [chunk*LA, chunk*LA + LA) of one pair (256 threads).
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
+__device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsigned char* smem_raw) {
   constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + M;
   const int tid = threadIdx.x;
   const int n = a.n;
-  const int pidx = blockIdx.y;
-  const int l0 = blockIdx.x * LA;
+  const int l0 = chunk * LA;
   const int nl = min(LA, a.n2 - l0);
-  const PairItem it = a.pairs[pidx];
   for (int i = tid; i < M; i += 256) tw[i] = a.tw[i];
-  const z64* pa = a.phi1 + (size_t)it.a * a.n3 + (size_t)l0 * n;
-  const z64* pb = a.phi2 + (size_t)it.b * a.n3 + (size_t)l0 * n;
+  const z64* pa = a.pa + (size_t)l0 * n;
+  const z64* pb = a.pb + (size_t)l0 * n;
   RowCol rc(tid, 256, n);
   for (int e = tid; e < nl * n; e += 256, rc.next()) {
     z64 v;
@@ -146,14 +187,20 @@ __global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
     }
   }
   __syncthreads();
-  xfwd_step2<T, P, Q>(a, buf, pidx, l0, nl, 0);
+  xfwd_step2<T, P, Q>(a, buf, l0, nl, 0);
   __syncthreads();
   if (act) {
 #pragma unroll
     for (int k1 = 0; k1 < P; ++k1) buf[line * pitch + k1 * Q + j2] = b[k1];
   }
   __syncthreads();
-  xfwd_step2<T, P, Q>(a, buf, pidx, l0, nl, 1);
+  xfwd_step2<T, P, Q>(a, buf, l0, nl, 1);
+}
+
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  body_xfwd<T, P, Q>(make_view(a, blockIdx.y, P * Q), blockIdx.x, smem_raw);
 }
 
 // ------------------------------------------------------------- y/z planes
@@ -170,21 +217,23 @@ __global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
 // The LDS offsets are runtime values, so the compiler cannot prove the
 // twiddle tables distinct from the workspaces and does not pin hoisted
 // twiddles in VGPRs across phases.
-template <typename T, int P, int Q>
-__global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
+// One (pair, kx) plane.  Threads at or beyond NT (a larger block in the
+// multi-set kernel) only take part in the barriers.
+// V is KArgs<T> (single set: the fields stay in the kernel-argument segment,
+// where the compiler can reload them instead of holding them in registers) or
+// ItemView<T>; base points at plane number `plane` (of 2m) of this pair.
+template <typename T, int P, int Q, bool GUARD, typename V>
+__device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, unsigned char* smem_raw) {
   constexpr int R = P * Q, NT = YZCfg<P, Q>::NT, H = YZCfg<P, Q>::H, G = NT / 2, pitch = R | 1, M = 2 * R;
   constexpr bool KEEP = sizeof(T) == 4;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* T2 = tw + a.lds_tw2;
   cplx<T>* Y = tw + a.lds_y;
   cplx<T>* Z2 = tw + a.lds_z2;
   cplx<T>* IN = tw + a.lds_in;
-  const int tid = threadIdx.x;
+  // with GUARD (blocks larger than NT) idle threads get no task and no loop element
+  const int tid = (!GUARD || threadIdx.x < NT) ? (int)threadIdx.x : (1 << 30);
   const int n = a.n, n2 = a.n2;
-  const int plane = blockIdx.x;
-  const int pidx = blockIdx.y;
-  cplx<T>* base = a.X + ((size_t)pidx * 2 * R + plane) * n2;
   const int cx = plane / R, posx = plane - cx * R;
   const int kx = 2 * (posx / Q + P * (posx % Q)) + cx;
   const int kkx = min(kx, M - kx);
@@ -196,7 +245,7 @@ __global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
   }
   const int yz = tid % n, yj = tid / n;            // y-line tasks: (z, j2) / (z, k1)
   const bool ya = yj < Q, yb = yj < P;
-  const int cz = tid / G, tg = tid - cz * G;       // z tasks: channel group, index in group
+  const int cz = GUARD ? (tid / G >= 1 ? 1 : 0) : tid / G, tg = tid - cz * G;  // z tasks: channel group, index in group
   const int zrr = tg % H, zj = tg / H;             // (row in half, j2) / (row in half, k1)
   if (KEEP) {  // the input plane stays in LDS for both ky channels
     RowCol rc(tid, NT, n);
@@ -356,30 +405,35 @@ __global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
     }
   }
   __syncthreads();
-  RowCol rc(tid, NT, n);
+  RowCol rc((!GUARD || tid < NT) ? tid : 0, NT, n);
   for (int e = tid; e < n2; e += NT, rc.next()) base[e] = Y[rc.row * pitch + rc.col];
 }
 
-// ---------------------------------------------------------------- x inverse
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
-  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
+__global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  body_yz<T, P, Q, false>(a, a.X + ((size_t)blockIdx.y * 2 * (P * Q) + blockIdx.x) * a.n2, blockIdx.x, smem_raw);
+}
+
+// ---------------------------------------------------------------- x inverse
+// x inverse of lines [chunk*LA, chunk*LA + LA) of one pair and the output
+// scatter (256 threads).
+template <typename T, int P, int Q>
+__device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsigned char* smem_raw) {
+  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + M;
   const int tid = threadIdx.x;
   const int n = a.n;
-  const int pidx = blockIdx.y;
-  const int l0 = blockIdx.x * LA;
+  const int l0 = chunk * LA;
   const int nl = min(LA, a.n2 - l0);
-  const PairItem it = a.pairs[pidx];
   for (int i = tid; i < M; i += 256) tw[i] = a.tw[i];
   const int line = tid % LA, jj = tid / LA;
   cplx<T> acc[P];
 #pragma unroll
   for (int j = 0; j < P; ++j) acc[j] = mk((T)0, (T)0);
   for (int c = 0; c < 2; ++c) {
-    const cplx<T>* src = a.X + ((size_t)pidx * 2 * R + (size_t)c * R) * a.n2 + l0;
+    const cplx<T>* src = a.X + (size_t)c * R * a.n2 + l0;
     RowCol rc(tid, 256, nl);
     for (int e = tid; e < R * nl; e += 256, rc.next()) buf[rc.col * pitch + rc.row] = src[(size_t)rc.row * a.n2 + rc.col];
     __syncthreads();
@@ -411,8 +465,8 @@ __global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
   }
   __syncthreads();
   // twoereduced(:, col) and the mirror column (SD/coreproject.f90:521,528)
-  z64* o = a.out + (size_t)it.col * a.n3 + (size_t)l0 * n;
-  z64* om = it.mcol >= 0 ? a.out + (size_t)it.mcol * a.n3 + (size_t)l0 * n : nullptr;
+  z64* o = a.out + (size_t)l0 * n;
+  z64* om = a.outm ? a.outm + (size_t)l0 * n : nullptr;
   RowCol rc(tid, 256, n);
   for (int e = tid; e < nl * n; e += 256, rc.next()) {
     const cplx<T> v = buf[rc.row * pitch + rc.col];
@@ -423,6 +477,12 @@ __global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
       om[e] = d;
     }
   }
+}
+
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  body_xinv<T, P, Q>(make_view(a, blockIdx.y, P * Q), blockIdx.x, smem_raw);
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -468,6 +528,270 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   if (prof) prof->end(SDV_STAGE_XINV, e0, st);
   return SDV_OK;
 }
+
+// ---------------------------------------------------------- multi-set batch
+// Persistent kernels over a batch of orbital sets of uneven grid sizes: a
+// static work list (one entry per line chunk or plane of every pair of every
+// set, sorted by cost so the round-robin stays balanced) is walked by a grid
+// sized to the machine; each entry dispatches on its set's circulant size.
+template <typename T>
+struct BSetDev {
+  const z64* phi;
+  z64* out;
+  const cplx<T>* tw;
+  const T* khat;
+  int n, m, conj_b, mirror_conj;
+};
+
+template <typename T>
+__device__ __forceinline__ ItemView<T> batch_view(const BSetDev<T>& s, const BItemDev& it, cplx<T>* X) {
+  ItemView<T> v;
+  v.n = s.n;
+  v.n2 = s.n * s.n;
+  v.n3 = s.n * s.n * s.n;
+  v.pa = s.phi + (size_t)it.a * v.n3;
+  v.pb = s.phi + (size_t)it.b * v.n3;
+  v.density_mode = 0;
+  v.conj_b = s.conj_b;
+  v.mirror_conj = s.mirror_conj;
+  v.X = X + it.xoff;
+  v.out = s.out + (size_t)it.col * v.n3;
+  v.outm = it.mcol >= 0 ? s.out + (size_t)it.mcol * v.n3 : nullptr;
+  v.tw = s.tw;
+  v.khat = s.khat;
+  const int R = s.m, M = 2 * R, pitch = R | 1, H = (R + 1) / 2;
+  v.lds_tw2 = M;
+  v.lds_y = M + R + ((M + R) & 1);
+  v.lds_z2 = v.lds_y + R * pitch + ((R * pitch) & 1);
+  v.lds_in = v.lds_z2 + H * pitch + ((H * pitch) & 1);
+  return v;
+}
+
+#define SDV_SIZE_SWITCH(m_, CALL) \
+  switch (m_) {                   \
+    case 9: CALL(3, 3); break;    \
+    case 12: CALL(4, 3); break;   \
+    case 16: CALL(4, 4); break;   \
+    case 21: CALL(7, 3); break;   \
+    case 25: CALL(5, 5); break;   \
+    case 27: CALL(9, 3); break;   \
+    case 32: CALL(8, 4); break;   \
+    case 36: CALL(6, 6); break;   \
+    case 45: CALL(9, 5); break;   \
+    case 48: CALL(8, 6); break;   \
+    case 54: CALL(9, 6); break;   \
+    case 63: CALL(9, 7); break;   \
+    case 64: CALL(8, 8); break;   \
+    case 72: CALL(9, 8); break;   \
+    case 81: CALL(9, 9); break;   \
+    default: break;               \
+  }
+
+// Separately allocated per-size entry points: the multi-set kernels call
+// these rather than inlining every size, so each body keeps its own register
+// allocation and the kernel needs only the largest.
+template <typename T, int P, int Q>
+__device__ __noinline__ void call_xfwd(const ItemView<T>* v, int sub, unsigned char* smem) {
+  body_xfwd<T, P, Q>(*v, sub, smem);
+}
+template <typename T, int P, int Q>
+__device__ __noinline__ void call_yz(const ItemView<T>* v, int sub, unsigned char* smem) {
+  body_yz<T, P, Q, true>(*v, v->X + (size_t)sub * v->n2, sub, smem);
+}
+template <typename T, int P, int Q>
+__device__ __noinline__ void call_xinv(const ItemView<T>* v, int sub, unsigned char* smem) {
+  body_xinv<T, P, Q>(*v, sub, smem);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_xfwd_multi(const BSetDev<T>* __restrict__ sets,
+                                                     const BItemDev* __restrict__ items,
+                                                     const BWork* __restrict__ work, int nwork, cplx<T>* X) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const BWork wk = work[w];
+    const BItemDev it = items[wk.item];
+    const BSetDev<T> st = sets[it.set];
+    const ItemView<T> v = batch_view(st, it, X);
+#define SDV_CALL_XFWD(P_, Q_) call_xfwd<T, P_, Q_>(&v, wk.sub, smem_raw)
+    SDV_SIZE_SWITCH(st.m, SDV_CALL_XFWD)
+#undef SDV_CALL_XFWD
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(768) k_yz_multi(const BSetDev<T>* __restrict__ sets,
+                                                   const BItemDev* __restrict__ items,
+                                                   const BWork* __restrict__ work, int nwork, cplx<T>* X) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const BWork wk = work[w];
+    const BItemDev it = items[wk.item];
+    const BSetDev<T> st = sets[it.set];
+    const ItemView<T> v = batch_view(st, it, X);
+#define SDV_CALL_YZ(P_, Q_) call_yz<T, P_, Q_>(&v, wk.sub, smem_raw)
+    SDV_SIZE_SWITCH(st.m, SDV_CALL_YZ)
+#undef SDV_CALL_YZ
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_xinv_multi(const BSetDev<T>* __restrict__ sets,
+                                                     const BItemDev* __restrict__ items,
+                                                     const BWork* __restrict__ work, int nwork, cplx<T>* X) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const BWork wk = work[w];
+    const BItemDev it = items[wk.item];
+    const BSetDev<T> st = sets[it.set];
+    const ItemView<T> v = batch_view(st, it, X);
+#define SDV_CALL_XINV(P_, Q_) call_xinv<T, P_, Q_>(&v, wk.sub, smem_raw)
+    SDV_SIZE_SWITCH(st.m, SDV_CALL_XINV)
+#undef SDV_CALL_XINV
+    __syncthreads();
+  }
+}
+
+// SDV_SIZE_SWITCH is itself the switch statement; unsupported m gives 0.
+static int lines_per_block(int m) {
+#define SDV_LA(P_, Q_) return XCfg<P_, Q_>::LA
+  SDV_SIZE_SWITCH(m, SDV_LA)
+#undef SDV_LA
+  return 0;
+}
+static int yz_threads(int m) {
+#define SDV_NT(P_, Q_) return YZCfg<P_, Q_>::NT
+  SDV_SIZE_SWITCH(m, SDV_NT)
+#undef SDV_NT
+  return 0;
+}
+
+BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int grid_mult, int ncu) {
+  BatchPlan bp;
+  bp.precision = precision;
+  const size_t es = precision == 0 ? sizeof(z64) : sizeof(c32);
+  long long xoff = 0;
+  size_t shx = 0, shy = 0;
+  int nty = 64;
+  struct Cost {
+    long long c;
+    BWork w;
+  };
+  std::vector<Cost> wx, wy;
+  for (int si = 0; si < (int)sets.size(); ++si) {
+    const BatchSetHost& s = sets[si];
+    const int n = s.spec->g.n, m = s.spec->g.m, R = m, M = 2 * m, pitch = R | 1, H = (R + 1) / 2;
+    const int LA = lines_per_block(m);
+    if (LA <= 0 || yz_threads(m) <= 0) {  // every context's m comes from choose_geom
+      bp.items.clear();
+      bp.wx.clear();
+      bp.wy.clear();
+      bp.grid_x = bp.grid_y = 0;
+      return bp;
+    }
+    shx = std::max(shx, (size_t)(M + LA * pitch) * es);
+    const size_t lds_in = (size_t)(M + R + ((M + R) & 1)) + R * pitch + ((R * pitch) & 1) + H * pitch + ((H * pitch) & 1);
+    shy = std::max(shy, (lds_in + (precision ? (size_t)n * n : 0)) * es);
+    nty = std::max(nty, yz_threads(m));
+    for (const PairItem& p : *s.items) {
+      BItemDev it;
+      it.set = si;
+      it.a = p.a;
+      it.b = p.b;
+      it.col = p.col;
+      it.mcol = p.mcol;
+      it.xoff = xoff;
+      const int ii = (int)bp.items.size();
+      bp.items.push_back(it);
+      xoff += (long long)2 * m * n * n;
+      const int nch = (n * n + LA - 1) / LA;
+      for (int c = 0; c < nch; ++c) wx.push_back(Cost{(long long)LA * m, BWork{ii, c}});
+      for (int pl = 0; pl < 2 * m; ++pl) wy.push_back(Cost{(long long)m * m * m, BWork{ii, pl}});
+    }
+  }
+  auto bycost = [](const Cost& a, const Cost& b) {
+    if (a.c != b.c) return a.c > b.c;
+    if (a.w.item != b.w.item) return a.w.item < b.w.item;
+    return a.w.sub < b.w.sub;
+  };
+  std::stable_sort(wx.begin(), wx.end(), bycost);
+  std::stable_sort(wy.begin(), wy.end(), bycost);
+  for (auto& c : wx) bp.wx.push_back(c.w);
+  for (auto& c : wy) bp.wy.push_back(c.w);
+  bp.x_elems = xoff;
+  bp.x_bytes = (size_t)xoff * es;
+  bp.shx = shx;
+  bp.shy = shy;
+  bp.nty = nty;
+  bp.grid_x = std::max(1, std::min((int)bp.wx.size(), grid_mult * ncu * 4));
+  bp.grid_y = std::max(1, std::min((int)bp.wy.size(), grid_mult * ncu));
+  return bp;
+}
+
+template <typename T>
+static int batch_launch_t(const BatchPlan& bp, const BatchDev& d, hipStream_t st, Prof* prof) {
+  SDV_REQUIRE(bp.shy <= 160 * 1024, SDV_ERR_UNSUPPORTED, "batch plane does not fit in LDS");
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz_multi<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)bp.shy));
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xfwd_multi<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)bp.shx));
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xinv_multi<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)bp.shx));
+  const BSetDev<T>* sets = (const BSetDev<T>*)d.sets;
+  hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
+  hipLaunchKernelGGL(k_xfwd_multi<T>, dim3(bp.grid_x), dim3(256), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
+                     (cplx<T>*)d.X);
+  SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
+  e0 = prof ? prof->begin(st) : nullptr;
+  hipLaunchKernelGGL(k_yz_multi<T>, dim3(bp.grid_y), dim3(bp.nty), bp.shy, st, sets, d.items, d.wy,
+                     (int)bp.wy.size(), (cplx<T>*)d.X);
+  SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_YZ, e0, st);
+  e0 = prof ? prof->begin(st) : nullptr;
+  hipLaunchKernelGGL(k_xinv_multi<T>, dim3(bp.grid_x), dim3(256), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
+                     (cplx<T>*)d.X);
+  SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_XINV, e0, st);
+  return SDV_OK;
+}
+
+int batch_launch(const BatchPlan& bp, const BatchDev& d, hipStream_t st, Prof* prof) {
+  return bp.precision == 0 ? batch_launch_t<double>(bp, d, st, prof) : batch_launch_t<float>(bp, d, st, prof);
+}
+
+size_t batch_set_bytes(int precision) { return precision == 0 ? sizeof(BSetDev<double>) : sizeof(BSetDev<float>); }
+
+void batch_fill_set(int precision, const BatchSetHost& s, void* dst) {
+  const int cn = s.cnorm;
+  if (precision == 0) {
+    BSetDev<double> b;
+    b.phi = (const z64*)s.phi;
+    b.out = (z64*)s.out;
+    b.tw = (const z64*)s.spec->tw64;
+    b.khat = s.spec->khat64;
+    b.n = s.spec->g.n;
+    b.m = s.spec->g.m;
+    b.conj_b = cn ? 0 : 1;
+    b.mirror_conj = cn ? 0 : 1;
+    std::memcpy(dst, &b, sizeof b);
+  } else {
+    BSetDev<float> b;
+    b.phi = (const z64*)s.phi;
+    b.out = (z64*)s.out;
+    b.tw = (const c32*)s.spec->tw32;
+    b.khat = s.spec->khat32;
+    b.n = s.spec->g.n;
+    b.m = s.spec->g.m;
+    b.conj_b = cn ? 0 : 1;
+    b.mirror_conj = cn ? 0 : 1;
+    std::memcpy(dst, &b, sizeof b);
+  }
+}
+size_t batch_item_bytes() { return sizeof(BItemDev); }
+size_t batch_work_bytes() { return sizeof(BWork); }
 
 // Supported half-edges m = P*Q (smallest m >= n is chosen).
 struct PQ {

@@ -74,6 +74,43 @@ struct ConvArgsHost {
 int conv_launch(const Spectrum& s, const ConvArgsHost& a, int precision, hipStream_t st, Prof* prof);
 size_t conv_work_bytes(const Geom& g, int npairs, int precision);
 
+// Multi-set batch (persistent kernels over uneven grid sizes).
+struct BatchSetHost {
+  const Spectrum* spec;
+  const std::vector<PairItem>* items;
+  const void* phi;  // z64 (n^3, N): bra and ket orbitals (same array)
+  void* out;        // z64 twoereduced (n^3, N, N)
+  int cnorm;
+};
+struct BItemDev {
+  int set, a, b, col, mcol;
+  long long xoff;  // element offset of this pair's planes in the batch buffer
+};
+struct BWork {
+  int item, sub;  // line chunk (x kernels) or kx plane (y/z kernel)
+};
+struct BatchPlan {
+  int precision = 0;
+  std::vector<BItemDev> items;
+  std::vector<BWork> wx, wy;
+  long long x_elems = 0;
+  size_t x_bytes = 0, shx = 0, shy = 0;
+  int nty = 64, grid_x = 1, grid_y = 1;
+};
+struct BatchDev {
+  const void* sets;
+  const BItemDev* items;
+  const BWork* wx;
+  const BWork* wy;
+  void* X;
+};
+BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int grid_mult, int ncu);
+int batch_launch(const BatchPlan& bp, const BatchDev& d, hipStream_t st, Prof* prof);
+size_t batch_set_bytes(int precision);
+void batch_fill_set(int precision, const BatchSetHost& s, void* dst);
+size_t batch_item_bytes();
+size_t batch_work_bytes();
+
 // Kernel construction (tooth + placement + spectrum), FP64.
 int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st);
 void free_spectrum(Spectrum* s);

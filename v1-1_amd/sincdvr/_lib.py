@@ -22,6 +22,8 @@ EXPORTS = [
     "sdv_get_reducedpot", "sdv_get_reducedpot_dev",
     "sdv_twoereduced", "sdv_twoereduced_dev", "sdv_profile", "sdv_profile_read", "sdv_nuclear_potential",
     "sdv_get_kernel_octant", "sdv_get_ke_stencil", "sdv_circulant_edge",
+    "sdv_batch_create", "sdv_batch_set_grid", "sdv_batch_run", "sdv_batch_capture", "sdv_batch_stats",
+    "sdv_batch_free", "sdv_batch_profile", "sdv_batch_profile_read",
 ]
 STAGES = ["xfwd", "yz", "xinv", "matel_assemble", "matel_gemm", "matel_reduce", "ke"]
 
@@ -42,6 +44,15 @@ class SdvConfig(ctypes.Structure):
         ("notwoflag", ctypes.c_int),
         ("debugflag", ctypes.c_int),
         ("precision", ctypes.c_int),
+    ]
+
+
+class SdvBatchSet(ctypes.Structure):
+    _fields_ = [
+        ("ctx", ctypes.c_void_p),
+        ("numspf", ctypes.c_int),
+        ("inspfs", ctypes.c_void_p),
+        ("twoereduced", ctypes.c_void_p),
     ]
 
 
@@ -88,6 +99,14 @@ def load():
     lib.sdv_get_kernel_octant.argtypes = [vp, vp]
     lib.sdv_get_ke_stencil.argtypes = [vp, vp]
     lib.sdv_circulant_edge.argtypes = [vp]
+    lib.sdv_batch_create.argtypes = [i, P(SdvBatchSet), i, P(vp)]
+    lib.sdv_batch_set_grid.argtypes = [vp, i]
+    lib.sdv_batch_run.argtypes = [vp, vp]
+    lib.sdv_batch_capture.argtypes = [vp, vp]
+    lib.sdv_batch_stats.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_longlong)]
+    lib.sdv_batch_free.argtypes = [vp]
+    lib.sdv_batch_profile.argtypes = [vp, i]
+    lib.sdv_batch_profile_read.argtypes = [vp, vp, vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype if name in (
             "sdv_last_error", "sdv_config_default") else ctypes.c_int

@@ -106,3 +106,41 @@ CONFIGS = {
 # Smooth exterior scaling of the 41^3 complex-scaled case: the ramp covers the
 # outer 8 points of each axis (|x| = 3.9 ... 6.0) at angle 0.3 rad.
 ECS_41 = dict(theta=0.3, stretch=1.0, dist=3.75, smooth=2.25)
+
+
+# Batched workload: 64 independent orbital sets, grid edges from this list at
+# spacing 0.3 bohr, 4..14 orbitals of 3 Gaussians each, 1..3 nuclei of charge
+# 1..3 within +-1.5 bohr.
+BATCH_EDGES = (25, 31, 41, 51, 61)
+
+
+def nucleus_shift(pos, n, spacing):
+    """Half-spacing offset of a nucleus nearest `pos` bohr on one axis, with the
+    parity SD/sincproject.f90:199-204 requires (centershift = n-1 mod 2); the
+    nucleus then sits at -centershift*spacing/2."""
+    cs = int(round(-2.0 * pos / spacing))
+    if (cs - (n - 1)) % 2:
+        cs += 1 if -2.0 * pos / spacing > cs else -1
+    return cs
+
+
+def batch_sets(nsets=64, seed=2000, spacing=0.3, edges=BATCH_EDGES, norb_range=(4, 14), ngauss=3):
+    """Descriptions of the orbital sets of the batched workload (orbitals are
+    made by batch_orbitals, so a caller can size things first)."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(nsets):
+        n = edges[rng.randint(0, len(edges) - 1)]
+        norb = rng.randint(*norb_range)
+        ncen = rng.randint(1, 3)
+        cs, z = [], []
+        for _ in range(ncen):
+            cs.append([nucleus_shift(rng.uniform(-1.5, 1.5), n, spacing) for _ in range(3)])
+            z.append(float(rng.randint(1, 3)))
+        out.append(dict(n=n, spacing=spacing, norb=norb, ngauss=ngauss, seed=seed * 1000 + i,
+                        centershift=cs, charges=z))
+    return out
+
+
+def batch_orbitals(desc, cnorm=False):
+    return gaussian_orbitals(desc["n"], desc["spacing"], desc["norb"], desc["ngauss"], desc["seed"], cnorm)

@@ -234,3 +234,75 @@ class SincDVR:
         out = np.zeros((self.totpoints, N, N), dtype=np.complex128, order="F")
         _lib.check(self._lib.sdv_get_reducedpot(self._h, _hptr(tr), _hptr(g), _hptr(out)))
         return out
+
+
+class Batch:
+    """Pair potentials of many orbital sets (each a SincDVR grid with its own
+    numspf) in three persistent launches per step.  `sets` is a list of
+    (SincDVR, inspfs) with inspfs a device tensor (numspf, totpoints); the
+    results are device tensors (numspf, numspf, totpoints) -- for each set the
+    twoereduced(totpoints, numspf, numspf) that call_twoe_matel00's
+    pair-symmetric path (lowspf..highspf = 1..numspf) produces."""
+
+    def __init__(self, sets, precision=0, grid_mult=None, outs=None):
+        lib = _lib.load()
+        self._lib = lib
+        self._keep = []
+        arr = (_lib.SdvBatchSet * len(sets))()
+        self.outputs = []
+        for i, (s, phi) in enumerate(sets):
+            phi = phi.contiguous()
+            assert phi.shape == (s.numspf, s.totpoints), (phi.shape, s.numspf, s.totpoints)
+            out = outs[i] if outs is not None else torch.zeros((s.numspf, s.numspf, s.totpoints),
+                                                               dtype=torch.complex128, device=phi.device)
+            self._keep.append((s, phi))
+            self.outputs.append(out)
+            arr[i].ctx = s._h
+            arr[i].numspf = s.numspf
+            arr[i].inspfs = _dptr(phi)
+            arr[i].twoereduced = _dptr(out)
+        h = ctypes.c_void_p()
+        _lib.check(lib.sdv_batch_create(len(sets), arr, int(precision), ctypes.byref(h)))
+        self._h = h
+        self.precision = precision
+        if grid_mult is not None:
+            self.set_grid(grid_mult)
+
+    def set_grid(self, grid_mult):
+        _lib.check(self._lib.sdv_batch_set_grid(self._h, int(grid_mult)))
+        self.grid_mult = grid_mult
+
+    def run(self, stream=None):
+        st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else _stream()
+        _lib.check(self._lib.sdv_batch_run(self._h, st))
+        return self.outputs
+
+    def capture(self, stream):
+        """Record one step as a hipGraph on `stream` (a non-default
+        torch.cuda.Stream); later run(stream) calls replay it."""
+        _lib.check(self._lib.sdv_batch_capture(self._h, ctypes.c_void_p(stream.cuda_stream)))
+
+    def stats(self):
+        npairs, wb = ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.check(self._lib.sdv_batch_stats(self._h, ctypes.byref(npairs), ctypes.byref(wb)))
+        return {"pairs": npairs.value, "work_bytes": wb.value}
+
+    def profile(self, enable=True):
+        _lib.check(self._lib.sdv_batch_profile(self._h, int(enable)))
+
+    def profile_read(self):
+        ms = (ctypes.c_double * len(_lib.STAGES))()
+        cnt = (ctypes.c_int * len(_lib.STAGES))()
+        _lib.check(self._lib.sdv_batch_profile_read(self._h, ms, cnt))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.STAGES)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sdv_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
