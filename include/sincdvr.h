@@ -117,8 +117,8 @@ int sdv_twoereduced_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspf
                         void* twoereduced, void* stream);
 
 /* Batched form: the pair potentials of many independent orbital sets, of
- * uneven grid sizes and orbital counts, in one persistent launch per pipeline
- * stage (three launches per step).  Each set is a context (its grid, kernel
+ * uneven grid sizes and orbital counts, as one step of a few launches (see
+ * the schedules below), capturable as one hipGraph.  Each set is a context (its grid, kernel
  * spectrum, numspf) with device arrays inspfs(totpoints, numspf) and
  * twoereduced(totpoints, numspf, numspf); the pair-symmetric path of
  * call_twoe_matel00 (one array as bra and ket, lowspf..highspf = 1..numspf)
@@ -126,8 +126,13 @@ int sdv_twoereduced_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspf
  * one orbital set per process); the per-set result equals that of
  * sdv_twoereduced_dev on the same set.
  * sdv_batch_capture records one step into a hipGraph on `stream`; later
- * sdv_batch_run calls on that stream replay it.  sdv_batch_set_grid sets the
- * persistent grid to grid_mult x (compute units) workgroups (autotuning). */
+ * sdv_batch_run calls on that stream replay it (a schedule change drops the
+ * graph).  sdv_batch_set_schedule picks the schedule (autotuning):
+ *   SDV_BATCH_GROUPED (default): one launch per stage per circulant size,
+ *     each with the block shape and LDS of its size; sizes run on up to
+ *     `nstreams` forked streams (parallel branches of the captured graph);
+ *   SDV_BATCH_FUSED: one persistent launch per stage over every set, grid of
+ *     grid_mult x (compute units) workgroups. */
 typedef struct sdv_batch sdv_batch;
 typedef struct sdv_batch_set {
   sdv_ctx* ctx;
@@ -136,7 +141,9 @@ typedef struct sdv_batch_set {
   void* twoereduced;
 } sdv_batch_set;
 int sdv_batch_create(int nsets, const sdv_batch_set* sets, int precision, sdv_batch** batch);
-int sdv_batch_set_grid(sdv_batch* batch, int grid_mult);
+#define SDV_BATCH_FUSED 0
+#define SDV_BATCH_GROUPED 1
+int sdv_batch_set_schedule(sdv_batch* batch, int mode, int grid_mult, int nstreams);
 int sdv_batch_run(sdv_batch* batch, void* stream);
 int sdv_batch_capture(sdv_batch* batch, void* stream);
 int sdv_batch_stats(sdv_batch* batch, long long* npairs, long long* work_bytes);

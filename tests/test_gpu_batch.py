@@ -1,5 +1,5 @@
 """GPU tests of the batched form: many orbital sets of uneven grid sizes and
-orbital counts in three persistent launches per step.
+orbital counts in one step (grouped or fused schedule).
 
 The batch runs the same per-pair arithmetic as the single-set path
 (sdv_twoereduced_dev), so each set's result must equal that path's result
@@ -51,13 +51,14 @@ def single_results(torch, sets, precision):
     return res
 
 
+@pytest.mark.parametrize("mode", ["grouped", "fused"])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_batch_equals_single_set(torch_dev, precision):
+def test_batch_equals_single_set(torch_dev, precision, mode):
     torch = torch_dev
     from sincdvr import Batch
     sets = make_sets(torch, SETS)
-    say("batch of %d sets precision=%d" % (len(sets), precision))
-    b = Batch([(s, p) for s, p, _ in sets], precision=precision)
+    say("batch of %d sets precision=%d schedule %s" % (len(sets), precision, mode))
+    b = Batch([(s, p) for s, p, _ in sets], precision=precision, schedule=(mode, 1, 3))
     st = b.stats()
     assert st["pairs"] == sum(N * (N + 1) // 2 for _, N, _ in SETS)
     outs = b.run()
@@ -85,15 +86,15 @@ def test_batch_fp64_vs_oracle(orc, torch_dev):
     b.close()
 
 
-@pytest.mark.parametrize("grid_mult", [1, 3])
-def test_batch_grid_schedules_agree(torch_dev, grid_mult):
+@pytest.mark.parametrize("schedule", [("fused", 1, 0), ("fused", 3, 0), ("grouped", 1, 0), ("grouped", 1, 2)])
+def test_batch_schedules_agree(torch_dev, schedule):
     torch = torch_dev
     from sincdvr import Batch
     sets = make_sets(torch, SETS[:5])
     b = Batch([(s, p) for s, p, _ in sets], precision=1)
     base = [o.clone() for o in b.run()]
-    say("grid_mult=%d" % grid_mult)
-    b.set_grid(grid_mult)
+    say("schedule %s" % (schedule,))
+    b.set_schedule(*schedule)
     for o in b.outputs:
         o.zero_()
     outs = b.run()
@@ -103,14 +104,16 @@ def test_batch_grid_schedules_agree(torch_dev, grid_mult):
     b.close()
 
 
-def test_batch_graph_replay(torch_dev):
+@pytest.mark.parametrize("mode", ["grouped", "fused"])
+def test_batch_graph_replay(torch_dev, mode):
     """Capture one step as a hipGraph, then replay it several times with
     eager work (rescaling the orbitals in place) and host synchronisation in
-    between; every replay must equal the eager result for the current input."""
+    between; every replay must equal the eager result for the current input.
+    The grouped schedule's sizes are parallel branches of the graph."""
     torch = torch_dev
     from sincdvr import Batch
     sets = make_sets(torch, SETS[2:6])
-    b = Batch([(s, p) for s, p, _ in sets], precision=1)
+    b = Batch([(s, p) for s, p, _ in sets], precision=1, schedule=(mode, 1, 3))
     side = torch.cuda.Stream()
     say("capture")
     b.capture(side)

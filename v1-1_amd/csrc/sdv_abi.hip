@@ -557,7 +557,11 @@ struct sdv_batch {
   std::vector<BatchSetHost> sets;
   BatchPlan plan;
   int precision = 0;
+  int mode = BATCH_GROUPED;
   int grid_mult = 1;
+  int nstreams = 4;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> ev;
   void* dsets = nullptr;
   void* ditems = nullptr;
   void* dwx = nullptr;
@@ -580,6 +584,10 @@ static void batch_release(sdv_batch* b) {
   (void)hipFree(b->dwy);
   (void)hipFree(b->X);
   b->dsets = b->ditems = b->dwx = b->dwy = b->X = nullptr;
+  for (hipStream_t st : b->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+  b->side.clear();
+  b->ev.clear();
 }
 
 static int batch_build(sdv_batch* b) {
@@ -587,7 +595,7 @@ static int batch_build(sdv_batch* b) {
   int dev = 0, ncu = 256;
   SDV_HIP_CHECK(hipGetDevice(&dev));
   SDV_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  b->plan = batch_plan(b->sets, b->precision, b->grid_mult, ncu);
+  b->plan = batch_plan(b->sets, b->precision, b->mode, b->grid_mult, ncu);
   SDV_REQUIRE(b->plan.grid_x > 0 && b->plan.grid_y > 0, SDV_ERR_UNSUPPORTED, "batch holds an unsupported grid");
   const size_t sb = batch_set_bytes(b->precision);
   std::vector<char> hs(sb * b->sets.size());
@@ -606,6 +614,19 @@ static int batch_build(sdv_batch* b) {
   SDV_HIP_CHECK(hipMemcpy(b->ditems, p.items.data(), p.items.size() * batch_item_bytes(), hipMemcpyHostToDevice));
   SDV_HIP_CHECK(hipMemcpy(b->dwx, p.wx.data(), p.wx.size() * batch_work_bytes(), hipMemcpyHostToDevice));
   SDV_HIP_CHECK(hipMemcpy(b->dwy, p.wy.data(), p.wy.size() * batch_work_bytes(), hipMemcpyHostToDevice));
+  if (b->mode == BATCH_GROUPED && p.groups.size() > 1) {
+    const int ns = std::min(b->nstreams, (int)p.groups.size());
+    for (int i = 0; i < ns; ++i) {
+      hipStream_t st = nullptr;
+      SDV_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      b->side.push_back(st);
+    }
+    for (int i = 0; i < ns + 1 && ns > 0; ++i) {
+      hipEvent_t e = nullptr;
+      SDV_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->ev.push_back(e);
+    }
+  }
   return SDV_OK;
 }
 
@@ -616,6 +637,9 @@ static BatchDev batch_dev(const sdv_batch* b) {
   d.wx = (const BWork*)b->dwx;
   d.wy = (const BWork*)b->dwy;
   d.X = b->X;
+  d.nside = (int)b->side.size();
+  d.side = d.nside ? const_cast<hipStream_t*>(b->side.data()) : nullptr;
+  d.ev = d.nside ? const_cast<hipEvent_t*>(b->ev.data()) : nullptr;
   return d;
 }
 
@@ -657,9 +681,15 @@ int sdv_batch_create(int nsets, const sdv_batch_set* sets, int precision, sdv_ba
   return SDV_OK;
 }
 
-int sdv_batch_set_grid(sdv_batch* b, int grid_mult) {
-  SDV_REQUIRE(b && grid_mult >= 1 && grid_mult <= 64, SDV_ERR_ARG, "grid multiplier must be in [1, 64]");
+int sdv_batch_set_schedule(sdv_batch* b, int mode, int grid_mult, int nstreams) {
+  SDV_REQUIRE(b, SDV_ERR_ARG, "null batch");
+  SDV_REQUIRE(mode == SDV_BATCH_FUSED || mode == SDV_BATCH_GROUPED, SDV_ERR_ARG, "unknown batch schedule");
+  SDV_REQUIRE(grid_mult >= 1 && grid_mult <= 64, SDV_ERR_ARG, "grid multiplier must be in [1, 64]");
+  SDV_REQUIRE(nstreams >= 0 && nstreams <= 8, SDV_ERR_ARG, "stream count must be in [0, 8]");
+  SDV_HIP_CHECK(hipDeviceSynchronize());
+  b->mode = mode;
   b->grid_mult = grid_mult;
+  b->nstreams = nstreams;
   return batch_build(b);
 }
 

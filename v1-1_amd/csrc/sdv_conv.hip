@@ -668,32 +668,74 @@ static int yz_threads(int m) {
   return 0;
 }
 
-BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int grid_mult, int ncu) {
+// Grouped schedule: one work entry per workgroup, block shape and LDS of
+// this circulant size (as the single-set kernels).
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__(256) k_xfwd_grp(const BSetDev<T>* __restrict__ sets,
+                                                   const BItemDev* __restrict__ items,
+                                                   const BWork* __restrict__ work, cplx<T>* X) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const BWork wk = work[blockIdx.x];
+  const BItemDev it = items[wk.item];
+  body_xfwd<T, P, Q>(batch_view(sets[it.set], it, X), wk.sub, smem_raw);
+}
+
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz_grp(const BSetDev<T>* __restrict__ sets,
+                                                               const BItemDev* __restrict__ items,
+                                                               const BWork* __restrict__ work, cplx<T>* X) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const BWork wk = work[blockIdx.x];
+  const BItemDev it = items[wk.item];
+  const ItemView<T> v = batch_view(sets[it.set], it, X);
+  body_yz<T, P, Q, false>(v, v.X + (size_t)wk.sub * v.n2, wk.sub, smem_raw);
+}
+
+template <typename T, int P, int Q>
+__global__ void __launch_bounds__(256) k_xinv_grp(const BSetDev<T>* __restrict__ sets,
+                                                   const BItemDev* __restrict__ items,
+                                                   const BWork* __restrict__ work, cplx<T>* X) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const BWork wk = work[blockIdx.x];
+  const BItemDev it = items[wk.item];
+  body_xinv<T, P, Q>(batch_view(sets[it.set], it, X), wk.sub, smem_raw);
+}
+
+// LDS of the x kernels and of the y/z kernel for half-edge m (elements of
+// cplx<T>; the y/z figure excludes the FP32 input-plane copy of n^2).
+static size_t lds_x_elems(int m) { return (size_t)2 * m + (size_t)lines_per_block(m) * (m | 1); }
+static size_t lds_yz_elems(int m) {
+  const size_t R = m, M = 2 * R, pitch = R | 1, H = (R + 1) / 2;
+  return (M + R + ((M + R) & 1)) + R * pitch + ((R * pitch) & 1) + H * pitch + ((H * pitch) & 1);
+}
+
+BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int mode, int grid_mult, int ncu) {
   BatchPlan bp;
   bp.precision = precision;
+  bp.mode = mode;
   const size_t es = precision == 0 ? sizeof(z64) : sizeof(c32);
   long long xoff = 0;
   size_t shx = 0, shy = 0;
   int nty = 64;
   struct Cost {
     long long c;
+    int m;
     BWork w;
   };
   std::vector<Cost> wx, wy;
+  std::vector<int> nmax(128, 0);  // largest grid edge per half-edge m (FP32 input-plane copy)
   for (int si = 0; si < (int)sets.size(); ++si) {
     const BatchSetHost& s = sets[si];
-    const int n = s.spec->g.n, m = s.spec->g.m, R = m, M = 2 * m, pitch = R | 1, H = (R + 1) / 2;
+    const int n = s.spec->g.n, m = s.spec->g.m;
     const int LA = lines_per_block(m);
-    if (LA <= 0 || yz_threads(m) <= 0) {  // every context's m comes from choose_geom
+    if (LA <= 0 || yz_threads(m) <= 0 || m >= 128) {  // every context's m comes from choose_geom
       bp.items.clear();
-      bp.wx.clear();
-      bp.wy.clear();
       bp.grid_x = bp.grid_y = 0;
       return bp;
     }
-    shx = std::max(shx, (size_t)(M + LA * pitch) * es);
-    const size_t lds_in = (size_t)(M + R + ((M + R) & 1)) + R * pitch + ((R * pitch) & 1) + H * pitch + ((H * pitch) & 1);
-    shy = std::max(shy, (lds_in + (precision ? (size_t)n * n : 0)) * es);
+    nmax[m] = std::max(nmax[m], n);
+    shx = std::max(shx, lds_x_elems(m) * es);
+    shy = std::max(shy, (lds_yz_elems(m) + (precision ? (size_t)n * n : 0)) * es);
     nty = std::max(nty, yz_threads(m));
     for (const PairItem& p : *s.items) {
       BItemDev it;
@@ -707,19 +749,47 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int g
       bp.items.push_back(it);
       xoff += (long long)2 * m * n * n;
       const int nch = (n * n + LA - 1) / LA;
-      for (int c = 0; c < nch; ++c) wx.push_back(Cost{(long long)LA * m, BWork{ii, c}});
-      for (int pl = 0; pl < 2 * m; ++pl) wy.push_back(Cost{(long long)m * m * m, BWork{ii, pl}});
+      for (int c = 0; c < nch; ++c) wx.push_back(Cost{(long long)LA * m, m, BWork{ii, c}});
+      for (int pl = 0; pl < 2 * m; ++pl) wy.push_back(Cost{(long long)m * m * m, m, BWork{ii, pl}});
     }
   }
-  auto bycost = [](const Cost& a, const Cost& b) {
+  // FUSED: largest cost first so the round-robin of the persistent grid stays
+  // balanced; GROUPED: by size (largest first), each size contiguous.
+  auto order = [mode](const Cost& a, const Cost& b) {
+    if (mode == BATCH_GROUPED && a.m != b.m) return a.m > b.m;
     if (a.c != b.c) return a.c > b.c;
     if (a.w.item != b.w.item) return a.w.item < b.w.item;
     return a.w.sub < b.w.sub;
   };
-  std::stable_sort(wx.begin(), wx.end(), bycost);
-  std::stable_sort(wy.begin(), wy.end(), bycost);
+  std::stable_sort(wx.begin(), wx.end(), order);
+  std::stable_sort(wy.begin(), wy.end(), order);
   for (auto& c : wx) bp.wx.push_back(c.w);
   for (auto& c : wy) bp.wy.push_back(c.w);
+  if (mode == BATCH_GROUPED) {
+    for (size_t i = 0; i < wy.size();) {
+      BatchGroup g;
+      g.m = wy[i].m;
+      g.wy0 = (int)i;
+      while (i < wy.size() && wy[i].m == g.m) ++i;
+      g.nwy = (int)i - g.wy0;
+      g.shx = lds_x_elems(g.m) * es;
+      g.shy = (lds_yz_elems(g.m) + (precision ? (size_t)nmax[g.m] * nmax[g.m] : 0)) * es;
+      bp.groups.push_back(g);
+    }
+    size_t gi = 0;
+    for (size_t i = 0; i < wx.size(); ++gi) {
+      const int m = wx[i].m;
+      const int b0 = (int)i;
+      while (i < wx.size() && wx[i].m == m) ++i;
+      if (gi >= bp.groups.size() || bp.groups[gi].m != m) {  // both lists hold the same sizes in the same order
+        bp.items.clear();
+        bp.grid_x = bp.grid_y = 0;
+        return bp;
+      }
+      bp.groups[gi].wx0 = b0;
+      bp.groups[gi].nwx = (int)i - b0;
+    }
+  }
   bp.x_elems = xoff;
   bp.x_bytes = (size_t)xoff * es;
   bp.shx = shx;
@@ -731,7 +801,7 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int g
 }
 
 template <typename T>
-static int batch_launch_t(const BatchPlan& bp, const BatchDev& d, hipStream_t st, Prof* prof) {
+static int batch_launch_fused(const BatchPlan& bp, const BatchDev& d, hipStream_t st, Prof* prof) {
   SDV_REQUIRE(bp.shy <= 160 * 1024, SDV_ERR_UNSUPPORTED, "batch plane does not fit in LDS");
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz_multi<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)bp.shy));
@@ -758,8 +828,63 @@ static int batch_launch_t(const BatchPlan& bp, const BatchDev& d, hipStream_t st
   return SDV_OK;
 }
 
+template <typename T, int P, int Q>
+static int launch_group(const BatchGroup& g, const BatchDev& d, hipStream_t st, Prof* prof) {
+  SDV_REQUIRE(g.shy <= 160 * 1024, SDV_ERR_UNSUPPORTED, "batch plane does not fit in LDS");
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz_grp<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)g.shy));
+  const BSetDev<T>* sets = (const BSetDev<T>*)d.sets;
+  cplx<T>* X = (cplx<T>*)d.X;
+  hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
+  hipLaunchKernelGGL((k_xfwd_grp<T, P, Q>), dim3(g.nwx), dim3(256), g.shx, st, sets, d.items, d.wx + g.wx0, X);
+  SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
+  e0 = prof ? prof->begin(st) : nullptr;
+  hipLaunchKernelGGL((k_yz_grp<T, P, Q>), dim3(g.nwy), dim3(YZCfg<P, Q>::NT), g.shy, st, sets, d.items,
+                     d.wy + g.wy0, X);
+  SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_YZ, e0, st);
+  e0 = prof ? prof->begin(st) : nullptr;
+  hipLaunchKernelGGL((k_xinv_grp<T, P, Q>), dim3(g.nwx), dim3(256), g.shx, st, sets, d.items, d.wx + g.wx0, X);
+  SDV_HIP_CHECK(hipGetLastError());
+  if (prof) prof->end(SDV_STAGE_XINV, e0, st);
+  return SDV_OK;
+}
+
+// Sizes run on forked streams (round robin), joined back into `st` at the
+// end; under stream capture the forks become parallel graph branches.  With
+// per-stage profiling on, everything stays on `st`.
+template <typename T>
+static int batch_launch_grouped(const BatchPlan& bp, const BatchDev& d, hipStream_t st, Prof* prof) {
+  const bool fork = d.nside > 0 && bp.groups.size() > 1 && !(prof && prof->on);
+  const int ns = fork ? std::min(d.nside, (int)bp.groups.size()) : 0;
+  if (fork) {
+    SDV_HIP_CHECK(hipEventRecord(d.ev[0], st));
+    for (int i = 0; i < ns; ++i) SDV_HIP_CHECK(hipStreamWaitEvent(d.side[i], d.ev[0], 0));
+  }
+  for (size_t gi = 0; gi < bp.groups.size(); ++gi) {
+    const BatchGroup& g = bp.groups[gi];
+    hipStream_t s = fork ? d.side[gi % ns] : st;
+    int rc = SDV_ERR_UNSUPPORTED;
+#define SDV_CALL_GRP(P_, Q_) rc = launch_group<T, P_, Q_>(g, d, s, fork ? nullptr : prof)
+    SDV_SIZE_SWITCH(g.m, SDV_CALL_GRP)
+#undef SDV_CALL_GRP
+    if (rc != SDV_OK) return rc;
+  }
+  if (fork) {
+    for (int i = 0; i < ns; ++i) {
+      SDV_HIP_CHECK(hipEventRecord(d.ev[1 + i], d.side[i]));
+      SDV_HIP_CHECK(hipStreamWaitEvent(st, d.ev[1 + i], 0));
+    }
+  }
+  return SDV_OK;
+}
+
 int batch_launch(const BatchPlan& bp, const BatchDev& d, hipStream_t st, Prof* prof) {
-  return bp.precision == 0 ? batch_launch_t<double>(bp, d, st, prof) : batch_launch_t<float>(bp, d, st, prof);
+  if (bp.mode == BATCH_GROUPED)
+    return bp.precision == 0 ? batch_launch_grouped<double>(bp, d, st, prof)
+                             : batch_launch_grouped<float>(bp, d, st, prof);
+  return bp.precision == 0 ? batch_launch_fused<double>(bp, d, st, prof) : batch_launch_fused<float>(bp, d, st, prof);
 }
 
 size_t batch_set_bytes(int precision) { return precision == 0 ? sizeof(BSetDev<double>) : sizeof(BSetDev<float>); }

@@ -89,10 +89,23 @@ struct BItemDev {
 struct BWork {
   int item, sub;  // line chunk (x kernels) or kx plane (y/z kernel)
 };
+// Work of one circulant size in the grouped schedule: ranges of the
+// size-sorted work lists and the LDS its kernels need.
+struct BatchGroup {
+  int m = 0;
+  int wx0 = 0, nwx = 0, wy0 = 0, nwy = 0;
+  size_t shx = 0, shy = 0;
+};
+// Schedules: FUSED = one persistent launch per stage over every set (block
+// shape and LDS of the largest size); GROUPED = one launch per stage per
+// circulant size, sizes on forked streams (branches of the captured graph).
+enum BatchMode { BATCH_FUSED = 0, BATCH_GROUPED = 1 };
 struct BatchPlan {
   int precision = 0;
+  int mode = BATCH_GROUPED;
   std::vector<BItemDev> items;
   std::vector<BWork> wx, wy;
+  std::vector<BatchGroup> groups;
   long long x_elems = 0;
   size_t x_bytes = 0, shx = 0, shy = 0;
   int nty = 64, grid_x = 1, grid_y = 1;
@@ -103,8 +116,11 @@ struct BatchDev {
   const BWork* wx;
   const BWork* wy;
   void* X;
+  int nside = 0;              // forked streams available to the grouped schedule
+  hipStream_t* side = nullptr;
+  hipEvent_t* ev = nullptr;   // 1 + nside events
 };
-BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int grid_mult, int ncu);
+BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int mode, int grid_mult, int ncu);
 int batch_launch(const BatchPlan& bp, const BatchDev& d, hipStream_t st, Prof* prof);
 size_t batch_set_bytes(int precision);
 void batch_fill_set(int precision, const BatchSetHost& s, void* dst);

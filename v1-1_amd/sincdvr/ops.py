@@ -244,7 +244,7 @@ class Batch:
     twoereduced(totpoints, numspf, numspf) that call_twoe_matel00's
     pair-symmetric path (lowspf..highspf = 1..numspf) produces."""
 
-    def __init__(self, sets, precision=0, grid_mult=None, outs=None):
+    def __init__(self, sets, precision=0, schedule=None, outs=None):
         lib = _lib.load()
         self._lib = lib
         self._keep = []
@@ -265,12 +265,17 @@ class Batch:
         _lib.check(lib.sdv_batch_create(len(sets), arr, int(precision), ctypes.byref(h)))
         self._h = h
         self.precision = precision
-        if grid_mult is not None:
-            self.set_grid(grid_mult)
+        self.schedule = ("grouped", 1, 4)
+        if schedule is not None:
+            self.set_schedule(*schedule)
 
-    def set_grid(self, grid_mult):
-        _lib.check(self._lib.sdv_batch_set_grid(self._h, int(grid_mult)))
-        self.grid_mult = grid_mult
+    def set_schedule(self, mode="grouped", grid_mult=1, nstreams=4):
+        """mode "grouped": one launch per stage per circulant size, sizes on
+        `nstreams` forked streams; "fused": one persistent launch per stage,
+        grid of grid_mult x compute units.  Drops a captured graph."""
+        m = {"fused": _lib.BATCH_FUSED, "grouped": _lib.BATCH_GROUPED}[mode]
+        _lib.check(self._lib.sdv_batch_set_schedule(self._h, m, int(grid_mult), int(nstreams)))
+        self.schedule = (mode, grid_mult, nstreams)
 
     def run(self, stream=None):
         st = ctypes.c_void_p(stream.cuda_stream) if stream is not None else _stream()
