@@ -116,6 +116,19 @@ int sdv_twoereduced(sdv_ctx* ctx, int lowspf, int highspf, const double* inspfs1
 int sdv_twoereduced_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs10, const void* inspfs20,
                         void* twoereduced, void* stream);
 
+/* Tuning (no counterpart in the reference, whose circulant edge is fixed at
+ * 2*numpoints, SD/coreproject.f90:389-391).  Any circulant edge M = 2m with
+ * m >= numpoints gives the same linear convolution; sdv_circulant_sizes lists
+ * the supported half-edges m (returns their count, writes at most cap).
+ * sdv_set_circulant rebuilds the kernel spectrum for half-edge m (0 = the
+ * smallest supported m >= numpoints); batches built on the context before the
+ * change must be rebuilt.  sdv_set_pair_batch sets how many pair potentials
+ * one launch of the convolution pipeline carries (work buffer = pairs x 2m x
+ * numpoints^2 complex). */
+int sdv_circulant_sizes(int* half_edges, int cap);
+int sdv_set_circulant(sdv_ctx* ctx, int half_edge);
+int sdv_set_pair_batch(sdv_ctx* ctx, int pairs);
+
 /* Batched form: the pair potentials of many independent orbital sets, of
  * uneven grid sizes and orbital counts, as one step of a few launches (see
  * the schedules below), capturable as one hipGraph.  Each set is a context (its grid, kernel

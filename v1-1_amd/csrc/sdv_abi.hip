@@ -133,6 +133,7 @@ struct sdv_ctx {
   int n = 0, n3 = 0, N = 0;
   int precision = 0;
   int pair_batch = 32;
+  int auto_m = 0;      // half-edge choose_geom picks for n (the default circulant)
   Spectrum spec;
   double* ke = nullptr;
   z64* invj = nullptr;
@@ -215,6 +216,7 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
     return SDV_ERR_HIP;
   }
   rc = build_spectrum(c->n, cfg->spacing, cfg->toothnbig, cfg->toothnsmall, &c->spec, c->stream);
+  c->auto_m = c->spec.g.m;
   if (rc == SDV_OK) {
     const int n = c->n;
     if (hipMalloc(&c->ke, (2 * n) * sizeof(double)) != hipSuccess) rc = SDV_ERR_NOMEM;
@@ -265,6 +267,32 @@ int sdv_set_precision(sdv_ctx* c, int p) {
 }
 
 int sdv_circulant_edge(sdv_ctx* c) { return c ? c->spec.g.M : SDV_ERR_ARG; }
+
+int sdv_circulant_sizes(int* half_edges, int cap) {
+  SDV_REQUIRE(cap >= 0 && (half_edges || cap == 0), SDV_ERR_ARG, "bad argument");
+  return circulant_sizes(half_edges, cap);
+}
+
+int sdv_set_circulant(sdv_ctx* c, int half_edge) {
+  SDV_REQUIRE(c, SDV_ERR_ARG, "null context");
+  if (half_edge != 0) {
+    Geom g;
+    SDV_REQUIRE(half_edge >= c->n && choose_geom(c->n, &g, half_edge) && g.m == half_edge, SDV_ERR_ARG,
+                "half edge must be a supported size >= numpoints");
+  }
+  if (half_edge == c->spec.g.m || (half_edge == 0 && c->spec.g.m == c->auto_m)) return SDV_OK;
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  SDV_HIP_CHECK(hipDeviceSynchronize());
+  free_spectrum(&c->spec);
+  c->spec = Spectrum();
+  return build_spectrum(c->n, c->cfg.spacing, c->cfg.toothnbig, c->cfg.toothnsmall, &c->spec, c->stream, half_edge);
+}
+
+int sdv_set_pair_batch(sdv_ctx* c, int pairs) {
+  SDV_REQUIRE(c && pairs >= 1, SDV_ERR_ARG, "pair batch must be >= 1");
+  c->pair_batch = pairs;
+  return SDV_OK;
+}
 
 int sdv_nuclear_potential(sdv_ctx* c, int numcenters, const int* centershift, const double* nuccharges,
                           int realkind4, double* pot) {

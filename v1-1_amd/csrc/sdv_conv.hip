@@ -926,9 +926,9 @@ static const PQ kSizes[] = {{9, 3, 3},  {12, 4, 3}, {16, 4, 4}, {21, 7, 3}, {25,
                             {27, 9, 3}, {32, 8, 4}, {36, 6, 6}, {45, 9, 5}, {48, 8, 6},
                             {54, 9, 6}, {63, 9, 7}, {64, 8, 8}, {72, 9, 8}, {81, 9, 9}};
 
-bool choose_geom(int n, Geom* g) {
+bool choose_geom(int n, Geom* g, int min_m) {
   for (const PQ& p : kSizes)
-    if (p.m >= n) {
+    if (p.m >= n && p.m >= min_m) {
       g->n = n;
       g->m = p.m;
       g->M = 2 * p.m;
@@ -937,6 +937,12 @@ bool choose_geom(int n, Geom* g) {
       return true;
     }
   return false;
+}
+
+int circulant_sizes(int* out, int cap) {
+  const int k = (int)(sizeof(kSizes) / sizeof(kSizes[0]));
+  for (int i = 0; i < k && i < cap; ++i) out[i] = kSizes[i].m;
+  return k;
 }
 
 size_t conv_work_bytes(const Geom& g, int npairs, int precision) {

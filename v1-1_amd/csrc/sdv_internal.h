@@ -43,7 +43,9 @@ struct PairItem {
 struct Geom {
   int n = 0, m = 0, M = 0, P = 0, Q = 0;
 };
-bool choose_geom(int n, Geom* g);
+// Smallest supported half-edge m >= max(n, min_m); false if none.
+bool choose_geom(int n, Geom* g, int min_m = 0);
+int circulant_sizes(int* out, int cap);
 
 // Device-resident tables for one geometry.
 struct Spectrum {
@@ -128,7 +130,7 @@ size_t batch_item_bytes();
 size_t batch_work_bytes();
 
 // Kernel construction (tooth + placement + spectrum), FP64.
-int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st);
+int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st, int min_m = 0);
 void free_spectrum(Spectrum* s);
 int nuclear_pot_launch(const Spectrum& s, double spacing, int ncent, const int* cs_dev, const double* z_dev,
                        int realkind4, double* pot_dev, hipStream_t st);

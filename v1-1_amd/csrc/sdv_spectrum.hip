@@ -254,10 +254,10 @@ int nuclear_pot_launch(const Spectrum& s, double spacing, int ncent, const int* 
 
 static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
-int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st) {
+int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st, int min_m) {
   SDV_REQUIRE(nbig > nsmall, SDV_ERR_ARG, "toothnbig must exceed toothnsmall (SD/tooth.f90:114-117)");
   SDV_REQUIRE(nbig >= n - 1, SDV_ERR_ARG, "toothnbig must be >= numpoints-1 (SD/sincproject.f90:151-153)");
-  SDV_REQUIRE(choose_geom(n, &s->g), SDV_ERR_UNSUPPORTED, "grid edge beyond the supported circulant sizes");
+  SDV_REQUIRE(choose_geom(n, &s->g, min_m), SDV_ERR_UNSUPPORTED, "grid edge beyond the supported circulant sizes");
   const Geom& g = s->g;
   const int NS = 2 * nsmall + 1;
   const long ns3 = (long)NS * NS * NS;

@@ -43,6 +43,14 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def circulant_sizes():
+    """Supported circulant half-edges m (circulant edge M = 2m)."""
+    lib = _lib.load()
+    buf = (ctypes.c_int * 64)()
+    k = lib.sdv_circulant_sizes(buf, 64)
+    return [buf[i] for i in range(min(k, 64))]
+
+
 class SincDVR:
     """One sinc-DVR grid with its device-resident Coulomb kernel spectrum, KE
     stencil and ECS tables (the project state of myprojectmod / myparams)."""
@@ -95,6 +103,15 @@ class SincDVR:
     @property
     def circulant_edge(self):
         return self._lib.sdv_circulant_edge(self._h)
+
+    def set_circulant(self, half_edge=0):
+        """Rebuild the kernel spectrum for circulant edge 2*half_edge (0: the
+        smallest supported half-edge >= numpoints)."""
+        _lib.check(self._lib.sdv_set_circulant(self._h, int(half_edge)))
+
+    def set_pair_batch(self, pairs):
+        _lib.check(self._lib.sdv_set_pair_batch(self._h, int(pairs)))
+        self.pair_batch = pairs
 
     def kernel_octant(self):
         """threed_two(i,j,k) for i,j,k in [0,n): (n,n,n) Fortran array."""
