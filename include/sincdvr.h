@@ -166,6 +166,39 @@ int sdv_batch_free(sdv_batch* batch);
 int sdv_batch_profile(sdv_batch* batch, int enable);
 int sdv_batch_profile_read(sdv_batch* batch, double* ms, int* count);
 
+/* Serving: a request engine for many concurrent propagations (no
+ * counterpart in the reference, which serves one propagation per process).
+ * Requests: SDV_REQ_COULOMB = the pair potentials twoereduced(totpoints,
+ * numspf, numspf) of one orbital set (as call_twoe_matel00's pair-symmetric
+ * path), SDV_REQ_KE = mult_ke of its numspf orbitals; inspfs is a device
+ * array (totpoints, numspf) that must stay unchanged until the request is
+ * done.  Outputs live in one device arena of hbm_budget bytes: a queued
+ * request is admitted (first in, first out) when its output fits, and its
+ * block returns to the arena at sdv_server_release.  Admitted pair work is
+ * cut into ticks of tick_bytes of convolution work buffer; one tick mixes
+ * pairs of several requests in one grouped launch (continuous batching), two
+ * ticks are in flight.  precision 1 = FP32 butterflies, 0 = FP64.
+ * sdv_server_poll retires finished ticks, admits and launches (wait = 1:
+ * until every admitted and admissible request is done); sdv_server_result
+ * gives the state (0 queued, 1 admitted, 2 done), the device output when done
+ * and the submit-to-done and submit-to-admit times in milliseconds.  A
+ * request's result is the same alone and in any mix.  Not thread-safe; the
+ * contexts must outlive their requests. */
+#define SDV_REQ_COULOMB 0
+#define SDV_REQ_KE 1
+typedef struct sdv_server sdv_server;
+int sdv_server_create(long long hbm_budget, long long tick_bytes, int precision, sdv_server** server);
+int sdv_server_submit(sdv_server* server, sdv_ctx* ctx, int kind, const void* inspfs, long long* request_id);
+int sdv_server_poll(sdv_server* server, int wait, int* ncompleted);
+int sdv_server_result(sdv_server* server, long long request_id, void** out, int* state, double* latency_ms,
+                      double* queue_ms);
+/* copy a done request's output (out_bytes of its kind) to device memory dst */
+int sdv_server_copy_result(sdv_server* server, long long request_id, void* dst, void* stream);
+int sdv_server_release(sdv_server* server, long long request_id);
+int sdv_server_stats(sdv_server* server, long long* queued, long long* in_flight, long long* arena_used,
+                     long long* arena_peak, long long* ticks);
+int sdv_server_free(sdv_server* server);
+
 /* Per-kernel timing with HIP events recorded on the launch stream, the
  * library's counterpart of the reference's twoematel.time.dat / zke.time.dat
  * counters (SD/coreproject.f90:1114-1142,1499-1517).  sdv_profile(ctx, 1)

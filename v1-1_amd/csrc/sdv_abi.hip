@@ -12,7 +12,9 @@
 #include <tuple>
 #include <vector>
 
-#include "sdv_internal.h"
+#include <mutex>
+
+#include "sdv_ctx.h"
 
 namespace sdv {
 
@@ -116,56 +118,61 @@ __global__ void k_assemble(int N, int nnn, const int2* __restrict__ rmap, const 
 
 using namespace sdv;
 
-// Index lists of one call shape, uploaded once and kept on the device: the
-// convolution items, and the row/column maps of the matrix-element assembly.
-struct IdxPlan {
-  std::vector<PairItem> items;
-  std::vector<int2> rows, rmap, cmap;
-  std::vector<int> cols;
-  void* dev = nullptr;
-  size_t off_items = 0, off_rows = 0, off_cols = 0, off_rmap = 0, off_cmap = 0;
+// ---- shared kernel spectra
+namespace {
+using SpecKey = std::tuple<int, int, long long, int, int, int>;  // device, n, spacing bits, nbig, nsmall, m
+struct SpecEntry {
+  sdv::Spectrum* s;
+  int refs;
 };
+std::mutex g_spec_mu;
+std::map<SpecKey, SpecEntry> g_specs;
+}  // namespace
 
-struct sdv_ctx {
-  sdv_config cfg;
-  Prof prof;
-  std::map<std::tuple<int, int, int, int>, IdxPlan> plans;  // (kind, low, high, sym)
-  int n = 0, n3 = 0, N = 0;
-  int precision = 0;
-  int pair_batch = 32;
-  int auto_m = 0;      // half-edge choose_geom picks for n (the default circulant)
-  Spectrum spec;
-  double* ke = nullptr;
-  z64* invj = nullptr;
-  z64* sdiag = nullptr;
-  hipStream_t stream = nullptr;
-  // growable device buffers
-  struct Buf {
-    void* p = nullptr;
-    size_t cap = 0;
-  };
-  Buf work, scratch, gout, h_in1, h_in2, h_out1, h_out2;
-};
-
-static int ensure(sdv_ctx::Buf& b, size_t bytes) {
-  if (b.cap >= bytes) return SDV_OK;
-  if (b.p) hipFree(b.p);
-  b.p = nullptr;
-  b.cap = 0;
-  if (hipMalloc(&b.p, bytes) != hipSuccess) {
-    b.p = nullptr;
-    set_error("device allocation of " + std::to_string(bytes) + " bytes failed");
-    return SDV_ERR_NOMEM;
+int sdv::spectrum_acquire(int n, double spacing, int nbig, int nsmall, int min_m, hipStream_t st, Spectrum** out) {
+  Geom g;
+  SDV_REQUIRE(choose_geom(n, &g, min_m), SDV_ERR_UNSUPPORTED, "grid edge beyond the supported circulant sizes");
+  int dev = 0;
+  SDV_HIP_CHECK(hipGetDevice(&dev));
+  long long bits = 0;
+  std::memcpy(&bits, &spacing, sizeof bits);
+  const SpecKey key(dev, n, bits, nbig, nsmall, g.m);
+  std::lock_guard<std::mutex> lk(g_spec_mu);
+  auto it = g_specs.find(key);
+  if (it != g_specs.end()) {
+    it->second.refs++;
+    *out = it->second.s;
+    return SDV_OK;
   }
-  b.cap = bytes;
+  Spectrum* s = new Spectrum();
+  int rc = build_spectrum(n, spacing, nbig, nsmall, s, st, min_m);
+  if (rc == SDV_OK && hipStreamSynchronize(st) != hipSuccess) {  // other streams and contexts use it next
+    set_error("spectrum construction failed");
+    rc = SDV_ERR_HIP;
+  }
+  if (rc != SDV_OK) {
+    free_spectrum(s);
+    delete s;
+    return rc;
+  }
+  g_specs.emplace(key, SpecEntry{s, 1});
+  *out = s;
   return SDV_OK;
 }
 
-#define SDV_TRY(expr)          \
-  do {                         \
-    int _rc = (expr);          \
-    if (_rc != SDV_OK) return _rc; \
-  } while (0)
+void sdv::spectrum_release(Spectrum* s) {
+  if (!s) return;
+  std::lock_guard<std::mutex> lk(g_spec_mu);
+  for (auto it = g_specs.begin(); it != g_specs.end(); ++it)
+    if (it->second.s == s) {
+      if (--it->second.refs == 0) {
+        free_spectrum(s);
+        delete s;
+        g_specs.erase(it);
+      }
+      return;
+    }
+}
 
 extern "C" {
 
@@ -215,8 +222,8 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
     delete c;
     return SDV_ERR_HIP;
   }
-  rc = build_spectrum(c->n, cfg->spacing, cfg->toothnbig, cfg->toothnsmall, &c->spec, c->stream);
-  c->auto_m = c->spec.g.m;
+  rc = spectrum_acquire(c->n, cfg->spacing, cfg->toothnbig, cfg->toothnsmall, 0, c->stream, &c->spec);
+  if (rc == SDV_OK) c->auto_m = c->spec->g.m;
   if (rc == SDV_OK) {
     const int n = c->n;
     if (hipMalloc(&c->ke, (2 * n) * sizeof(double)) != hipSuccess) rc = SDV_ERR_NOMEM;
@@ -243,7 +250,7 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
 int sdv_free(sdv_ctx* c) {
   if (!c) return SDV_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
-  free_spectrum(&c->spec);
+  spectrum_release(c->spec);
   hipFree(c->ke);
   hipFree(c->invj);
   hipFree(c->sdiag);
@@ -266,7 +273,7 @@ int sdv_set_precision(sdv_ctx* c, int p) {
   return SDV_OK;
 }
 
-int sdv_circulant_edge(sdv_ctx* c) { return c ? c->spec.g.M : SDV_ERR_ARG; }
+int sdv_circulant_edge(sdv_ctx* c) { return c && c->spec ? c->spec->g.M : SDV_ERR_ARG; }
 
 int sdv_circulant_sizes(int* half_edges, int cap) {
   SDV_REQUIRE(cap >= 0 && (half_edges || cap == 0), SDV_ERR_ARG, "bad argument");
@@ -280,12 +287,14 @@ int sdv_set_circulant(sdv_ctx* c, int half_edge) {
     SDV_REQUIRE(half_edge >= c->n && choose_geom(c->n, &g, half_edge) && g.m == half_edge, SDV_ERR_ARG,
                 "half edge must be a supported size >= numpoints");
   }
-  if (half_edge == c->spec.g.m || (half_edge == 0 && c->spec.g.m == c->auto_m)) return SDV_OK;
+  if (half_edge == c->spec->g.m || (half_edge == 0 && c->spec->g.m == c->auto_m)) return SDV_OK;
   SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
   SDV_HIP_CHECK(hipDeviceSynchronize());
-  free_spectrum(&c->spec);
-  c->spec = Spectrum();
-  return build_spectrum(c->n, c->cfg.spacing, c->cfg.toothnbig, c->cfg.toothnsmall, &c->spec, c->stream, half_edge);
+  Spectrum* ns = nullptr;
+  SDV_TRY(spectrum_acquire(c->n, c->cfg.spacing, c->cfg.toothnbig, c->cfg.toothnsmall, half_edge, c->stream, &ns));
+  spectrum_release(c->spec);
+  c->spec = ns;
+  return SDV_OK;
 }
 
 int sdv_set_pair_batch(sdv_ctx* c, int pairs) {
@@ -319,7 +328,7 @@ int sdv_nuclear_potential(sdv_ctx* c, int numcenters, const int* centershift, co
         hipMemcpy(dz, nuccharges, numcenters * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       rc = SDV_ERR_HIP;
   }
-  if (rc == SDV_OK) rc = nuclear_pot_launch(c->spec, c->cfg.spacing, numcenters, dcs, dz, realkind4, dpot, c->stream);
+  if (rc == SDV_OK) rc = nuclear_pot_launch(*c->spec, c->cfg.spacing, numcenters, dcs, dz, realkind4, dpot, c->stream);
   if (rc == SDV_OK && (hipMemcpyAsync(pot, dpot, n3 * sizeof(double), hipMemcpyDefault, c->stream) != hipSuccess ||
                        hipStreamSynchronize(c->stream) != hipSuccess))
     rc = SDV_ERR_HIP;
@@ -332,7 +341,7 @@ int sdv_nuclear_potential(sdv_ctx* c, int numcenters, const int* centershift, co
 
 int sdv_get_kernel_octant(sdv_ctx* c, double* koct) {
   SDV_REQUIRE(c && koct, SDV_ERR_ARG, "null argument");
-  SDV_HIP_CHECK(hipMemcpy(koct, c->spec.koct, (size_t)c->n3 * sizeof(double), hipMemcpyDeviceToHost));
+  SDV_HIP_CHECK(hipMemcpy(koct, c->spec->koct, (size_t)c->n3 * sizeof(double), hipMemcpyDeviceToHost));
   return SDV_OK;
 }
 
@@ -341,6 +350,8 @@ int sdv_get_ke_stencil(sdv_ctx* c, double* ketot) {
   SDV_HIP_CHECK(hipMemcpy(ketot, c->ke, (size_t)(2 * c->n - 1) * sizeof(double), hipMemcpyDeviceToHost));
   return SDV_OK;
 }
+
+}  // extern "C"
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -375,7 +386,7 @@ static int upload_plan(IdxPlan& pl) {
 // orbitals each unordered pair is convolved once; its mirror column gets
 // conj(V) (hermnorm) or V (cnorm), and rows (1a,1b) with 1a > 1b reuse the
 // density of (1b,1a) the same way.
-static int twoe_plan(sdv_ctx* c, int low, int high, bool sym, IdxPlan** out) {
+int sdv::twoe_plan(sdv_ctx* c, int low, int high, bool sym, IdxPlan** out) {
   auto key = std::make_tuple(1, low, high, sym ? 1 : 0);
   auto it = c->plans.find(key);
   if (it != c->plans.end()) {
@@ -446,7 +457,7 @@ static int run_pairs(sdv_ctx* c, const IdxPlan& pl, const void* phi1, const void
   const int np = (int)pl.items.size();
   if (np == 0) return SDV_OK;
   const int batch = std::min(np, c->pair_batch);
-  SDV_TRY(ensure(c->work, conv_work_bytes(c->spec.g, batch, c->precision)));
+  SDV_TRY(ensure(c->work, conv_work_bytes(c->spec->g, batch, c->precision)));
   for (int p0 = 0; p0 < np; p0 += batch) {
     ConvArgsHost a;
     a.n = c->n;
@@ -459,10 +470,12 @@ static int run_pairs(sdv_ctx* c, const IdxPlan& pl, const void* phi1, const void
     a.pairs_dev = (const PairItem*)((char*)pl.dev + pl.off_items) + p0;
     a.work = c->work.p;
     a.out = out;
-    SDV_TRY(conv_launch(c->spec, a, c->precision, st, &c->prof));
+    SDV_TRY(conv_launch(*c->spec, a, c->precision, st, &c->prof));
   }
   return SDV_OK;
 }
+
+extern "C" {
 
 int sdv_op_tinv_dev(sdv_ctx* c, const void* den, void* red, int allsize, int circsize, void* stream) {
   SDV_REQUIRE(c && den && red, SDV_ERR_ARG, "null argument");
@@ -692,7 +705,7 @@ int sdv_batch_create(int nsets, const sdv_batch_set* sets, int precision, sdv_ba
       return rc;
     }
     BatchSetHost h;
-    h.spec = &c->spec;
+    h.spec = c->spec;
     h.items = &pl->items;
     h.phi = sets[i].inspfs;
     h.out = sets[i].twoereduced;

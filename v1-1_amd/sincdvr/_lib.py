@@ -25,8 +25,12 @@ EXPORTS = [
     "sdv_circulant_sizes", "sdv_set_circulant", "sdv_set_pair_batch",
     "sdv_batch_create", "sdv_batch_set_schedule", "sdv_batch_run", "sdv_batch_capture", "sdv_batch_stats",
     "sdv_batch_free", "sdv_batch_profile", "sdv_batch_profile_read",
+    "sdv_server_create", "sdv_server_submit", "sdv_server_poll", "sdv_server_result", "sdv_server_copy_result",
+    "sdv_server_release",
+    "sdv_server_stats", "sdv_server_free",
 ]
 BATCH_FUSED, BATCH_GROUPED = 0, 1
+REQ_COULOMB, REQ_KE = 0, 1
 STAGES = ["xfwd", "yz", "xinv", "matel_assemble", "matel_gemm", "matel_reduce", "ke"]
 
 
@@ -111,6 +115,15 @@ def load():
     lib.sdv_batch_stats.argtypes = [vp, P(ctypes.c_longlong), P(ctypes.c_longlong)]
     lib.sdv_batch_free.argtypes = [vp]
     lib.sdv_batch_profile.argtypes = [vp, i]
+    ll = ctypes.c_longlong
+    lib.sdv_server_create.argtypes = [ll, ll, i, P(vp)]
+    lib.sdv_server_submit.argtypes = [vp, vp, i, vp, P(ll)]
+    lib.sdv_server_poll.argtypes = [vp, i, P(i)]
+    lib.sdv_server_result.argtypes = [vp, ll, P(vp), P(i), P(d), P(d)]
+    lib.sdv_server_copy_result.argtypes = [vp, ll, vp, vp]
+    lib.sdv_server_release.argtypes = [vp, ll]
+    lib.sdv_server_stats.argtypes = [vp, P(ll), P(ll), P(ll), P(ll), P(ll)]
+    lib.sdv_server_free.argtypes = [vp]
     lib.sdv_batch_profile_read.argtypes = [vp, vp, vp]
     for name in EXPORTS:
         getattr(lib, name).restype = getattr(lib, name).restype if name in (

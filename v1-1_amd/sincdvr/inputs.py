@@ -144,3 +144,31 @@ def batch_sets(nsets=64, seed=2000, spacing=0.3, edges=BATCH_EDGES, norb_range=(
 
 def batch_orbitals(desc, cnorm=False):
     return gaussian_orbitals(desc["n"], desc["spacing"], desc["norb"], desc["ngauss"], desc["seed"], cnorm)
+
+
+def gaussian_orbitals_device(n, spacing, norb, ngauss, seed, device="cuda"):
+    """gaussian_orbitals' recipe (same SplitMix64 draws) evaluated with torch on
+    the device, orthonormalised there (two-pass classical Gram-Schmidt, hermnorm
+    dot); returns (norb, n^3) complex128.  Agrees with gaussian_orbitals to
+    rounding; for load generation where the host recipe is too slow."""
+    import torch
+    rng = SplitMix64(seed)
+    x = torch.as_tensor(grid_points(n, spacing), dtype=torch.float64, device=device)
+    Z, Y, X = torch.meshgrid(x, x, x, indexing="ij")   # row-major (z, y, x): x fastest
+    out = torch.zeros((norb, n ** 3), dtype=torch.complex128, device=device)
+    for o in range(norb):
+        f = torch.zeros((n, n, n), dtype=torch.complex128, device=device)
+        for _ in range(ngauss):
+            alpha = rng.uniform(0.5, 2.0)
+            cx, cy, cz = rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1)
+            c = complex(rng.normal(), rng.normal())
+            f += c * torch.exp(-alpha * ((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2))
+        out[o] = f.reshape(-1)
+    for k in range(norb):
+        v = out[k].clone()
+        for _ in range(2):
+            if k:
+                v = v - torch.mv(out[:k].transpose(0, 1), torch.mv(out[:k].conj(), v))
+            v = v / torch.sqrt(torch.vdot(v, v).real)
+        out[k] = v
+    return out
