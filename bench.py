@@ -4,12 +4,25 @@ Workload: 81^3 sinc-DVR grid, spacing 0.25 bohr, 16 hermnorm orbitals (each a
 seeded sum of 6 Gaussians, Gram-Schmidt orthonormalised) -> 136 unique / 256
 ordered orbital pairs, circulant edge M = 162.  A step delivers the full
 twoereduced(81^3, 16, 16) array (every pair potential, mirrors filled) from
-orbitals already resident in HBM.  `value` is the plain FP64 form; the mixed
-form (FP32 butterflies, FP64 spectrum construction and reductions) is reported
-beside it with its measured error against FP64.  Two-electron matrix
-elements and the kinetic apply are timed separately.
+orbitals already resident in HBM.  `value` is the plain FP64 form; beside it:
+  * mixed: FP32 butterflies, FP64 spectrum construction and reductions, with
+    its two-electron matrix-element error against FP64 per element;
+  * batched_mixed_81: the same pairs through the batch API (one hipGraph per
+    step), mixed precision;
+  * batch: 64 orbital sets of uneven grids (edges 25..61, 4..14 orbitals),
+    FP32 and FP64, grouped schedule, one hipGraph per step;
+  * serving: Poisson request streams from 256 simulated propagations (edges
+    31..81, 6..16 orbitals) against the request engine, mixed and FP64, with
+    p50/p99 latency at 70% of each form's measured capacity;
+  * two-electron matrix elements and the kinetic apply, timed separately.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Roofline: algorithmic bytes per pair follow the pencil model
+B = s (2 n^3 + 4 M n^2 + 4 M^2 n) (s = 16 FP64, 8 FP32), split over the
+kernels by the passes each performs: k_xfwd = x-forward (R n^3, W M n^2),
+k_yz = y-forward + z (fused multiply) + y-inverse (R+W 2 M n^2 + 4 M^2 n),
+k_xinv = x-inverse (R M n^2, W n^3).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
 import argparse
 import json
@@ -25,17 +38,16 @@ METRIC = "pair Coulomb potentials/s at 81^3 (mixed vs FP64), % HBM peak, p99 lat
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes_per_pair(stage, n, m, s_elem, mirror_frac):
-    """Minimum HBM bytes one pair moves through each stage of the pipeline
-    (complex elements of s_elem bytes for the intermediate planes, 16 bytes
-    for FP64 orbitals and outputs)."""
-    plane_set = 2 * m * n * n * s_elem          # X[pair][kx][z][y], 2m planes of n^2
+def alg_bytes_per_pair(stage, n, M, s_elem):
+    """Pencil-model HBM bytes of one pair potential in each kernel's passes."""
     if stage == "xfwd":
-        return 2 * n ** 3 * 16 + plane_set       # read two orbitals, write planes
+        return s_elem * (n ** 3 + M * n * n)
     if stage == "yz":
-        return 2 * plane_set                     # read + write planes
+        return s_elem * (2 * M * n * n + 4 * M * M * n)
     if stage == "xinv":
-        return plane_set + (1 + mirror_frac) * n ** 3 * 16
+        return s_elem * (M * n * n + n ** 3)
+    if stage == "path":
+        return s_elem * (2 * n ** 3 + 4 * M * n * n + 4 * M * M * n)
     raise ValueError(stage)
 
 
@@ -50,11 +62,12 @@ def time_steps(fn, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(n, dx, phi, cores):
-    """The oracle (a C port of the reference's CPU path) on one spf2b batch
-    of call_twoe_matel00 -- 16 densities convolved by 3 x (1-D FFT +
-    transpose) at M = 2n = 162 with the kernel FFT recomputed, as the
-    reference does -- scaled to the full twoereduced (16 batches)."""
+def cpu_baseline(n, dx, phi, cores, nbatch=8):
+    """The oracle (a C port of the reference's CPU path) on `nbatch` spf2b
+    batches of call_twoe_matel00 -- each N densities phi(:,a)*conj(phi(:,b))
+    convolved by 3 x (1-D FFT + transpose) at M = 2n = 162 with the kernel FFT
+    recomputed, as the reference does -- scaled to the full twoereduced (N
+    batches)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
@@ -62,15 +75,105 @@ def cpu_baseline(n, dx, phi, cores):
     oracle.set_threads(cores)
     kern = oracle.coulomb_kernel(n, dx)
     N = phi.shape[1]
-    den = np.asfortranarray(phi * np.conj(phi[:, [0]]))   # densities (:, a) * conj(phi(:, 1))
-    t0 = time.perf_counter()
-    oracle.op_tinv(n, kern, den)
-    t = time.perf_counter() - t0
-    full = t * N                                          # N batches deliver all N^2 ordered pairs
+    nbatch = min(nbatch, N)
+    t = 0.0
+    for b in range(nbatch):
+        den = np.asfortranarray(phi * np.conj(phi[:, [b]]))
+        t0 = time.perf_counter()
+        oracle.op_tinv(n, kern, den)
+        t += time.perf_counter() - t0
+    full = t * N / nbatch
     return {"value": (N * (N + 1) // 2) / full, "unit": "pair Coulomb potentials/s", "cores": cores,
             "kind": "port",
-            "sample": "one spf2b batch of call_twoe_matel00 at 81^3 (16 densities + kernel FFT, M=162, "
-                      "%.1f s), scaled x16 to the full twoereduced" % t}
+            "sample": "%d of the %d spf2b batches of call_twoe_matel00 at 81^3 (each %d densities + the kernel "
+                      "FFT, M=162), %.1f s, scaled to the full twoereduced" % (nbatch, N, N, t)}
+
+
+def bench_batched_81(torch, s, phi, npairs, steps):
+    """The 81^3 / 16-orbital pairs through the batch API, mixed precision,
+    replayed as one hipGraph per step."""
+    from sincdvr import Batch
+    b = Batch([(s, phi)], precision=1)
+    side = torch.cuda.Stream()
+    b.capture(side)
+    for _ in range(2):
+        b.run(side)
+    side.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(side)
+    side.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    b.close()
+    return {"value": npairs / t, "ms_per_step": t * 1e3, "precision": "f32 butterflies", "graph": True}
+
+
+def bench_batch(torch, quick):
+    """64 orbital sets of uneven grids, grouped schedule, one hipGraph per step."""
+    from sincdvr import Batch, SincDVR, inputs
+    descs = inputs.batch_sets(nsets=16 if quick else 64)
+    sets = []
+    for d in descs:
+        phi = inputs.gaussian_orbitals_device(d["n"], d["spacing"], d["norb"], d["ngauss"], d["seed"])
+        sets.append((SincDVR(d["n"], d["spacing"], d["norb"]), phi))
+    npairs = sum(d["norb"] * (d["norb"] + 1) // 2 for d in descs)
+    out = {"sets": len(descs), "unique_pairs": npairs, "grid_edges": sorted({d["n"] for d in descs}),
+           "schedule": "grouped (one launch per stage per circulant size), hipGraph per step"}
+    for prec, name in ((1, "mixed"), (0, "fp64")):
+        b = Batch(sets, precision=prec)
+        side = torch.cuda.Stream()
+        b.capture(side)
+        b.run(side)
+        side.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run(side)
+        side.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        out[name] = {"pairs_per_s": npairs / t, "ms_per_step": t * 1e3,
+                     "work_buffer_gb": b.stats()["work_bytes"] / 1e9}
+        b.close()
+    del sets
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_serving(torch, quick):
+    """Request engine under Poisson load from simulated propagations."""
+    from sincdvr import serving
+    nprop = 32 if quick else 256
+    descs = serving.propagations(nprop=nprop)
+    ctxs, props = serving.build_propagations(descs, steps=2)
+    budget, tick = 48 << 30, 4 << 30
+    res = {"propagations": nprop, "grid_edges": list(serving.SERVING_EDGES), "orbitals": "6..16",
+           "hbm_budget_gb": budget / 2 ** 30, "tick_work_gb": tick / 2 ** 30,
+           "requests": "alternating Coulomb (all pair potentials) and kinetic apply per propagation",
+           "latency": "submit to device result ready, observed by the host, ms"}
+    for prec, name in ((1, "mixed"), (0, "fp64")):
+        srv = serving.Server(budget, tick, precision=prec)
+        # capacity: every request submitted at once
+        burst = serving.request_stream(props, nreq=nprop, rate=1e9, seed=11)
+        serving.run_stream(srv, props, burst[: min(16, len(burst))])          # warm-up
+        lat, q, wall = serving.run_stream(srv, props, burst)
+        cap = len(burst) / wall
+        pairs = serving.pair_count(props, burst)
+        rate = 0.7 * cap
+        nreq = 2 * nprop
+        stream = serving.request_stream(props, nreq=nreq, rate=rate, seed=12)
+        lat, q, wall = serving.run_stream(srv, props, stream)
+        ent = {"capacity_requests_per_s": cap, "capacity_pair_potentials_per_s": pairs / (len(burst) / cap),
+               "offered_requests_per_s": rate, "requests": nreq, "achieved_requests_per_s": nreq / wall,
+               "pair_potentials_per_s": serving.pair_count(props, stream) / wall,
+               "coulomb_latency_ms": serving.percentiles(lat[serving.COULOMB]),
+               "kinetic_latency_ms": serving.percentiles(lat[serving.KE]),
+               "queue_ms": serving.percentiles(q), "arena_peak_gb": srv.stats()["arena_peak"] / 2 ** 30,
+               "ticks": srv.stats()["ticks"]}
+        res[name] = ent
+        srv.close()
+    del props, ctxs
+    torch.cuda.empty_cache()
+    return res
 
 
 def main():
@@ -79,6 +182,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="smaller batch and serving sections")
+    ap.add_argument("--no-extra", action="store_true", help="headline forms only")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md",
@@ -98,10 +203,9 @@ def main():
     phi = torch.from_numpy(np.ascontiguousarray(phi_h.T)).cuda()
     print("[bench] building kernel spectrum", file=sys.stderr, flush=True)
     s = SincDVR(n, dx, N, precision=0)
-    m = s.circulant_edge // 2
+    M = s.circulant_edge
     tr = torch.zeros((N, N, n ** 3), dtype=torch.complex128, device="cuda")
     tm = torch.zeros((N, N, N, N), dtype=torch.complex128, device="cuda")
-    mirror_frac = (npairs - N) / npairs
 
     def run_form(prec):
         s.set_precision(prec)
@@ -134,14 +238,12 @@ def main():
     err = {"max_abs_over_max": float(d.max() / tm64.abs().max()),
            "max_rel_elem": float(rel_elem[big].max()),
            "median_rel_elem": float(rel_elem[big].median()),
+           "p99_rel_elem": float(torch.quantile(rel_elem[big].flatten()[:2 ** 24], 0.99)),
+           "elements_counted": int(big.sum()), "elements": int(tm64.numel()),
            "twoereduced_max_rel": float((tr - tr64).abs().max() / tr64.abs().max())}
+    del tr64
     print("[bench] kinetic apply", file=sys.stderr, flush=True)
-    ke_out = torch.empty_like(phi)
-    s.profile(True)
     tke = time_steps(lambda: s.mult_ke(phi), args.steps, 1, torch)
-    kprof = s.profile_read()
-    s.profile(False)
-    del ke_out
 
     # roofline of the dominant kernel of the FP64 form
     stages = ["xfwd", "yz", "xinv"]
@@ -149,14 +251,16 @@ def main():
     ms_tot, launches = prof64[dom]
     avg_ms = ms_tot / launches
     pairs_per_launch = npairs * args.steps / launches
-    alg = alg_bytes_per_pair(dom, n, m, 16, mirror_frac) * pairs_per_launch
+    alg = alg_bytes_per_pair(dom, n, M, 16) * pairs_per_launch
     achieved = alg / (avg_ms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_k_yz_81.json")
     if dom == "yz" and os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
-    path_bytes = sum(alg_bytes_per_pair(k, n, m, 16, mirror_frac) for k in stages) * npairs
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch")
+        if traffic is not None and tj.get("pairs_per_launch"):
+            traffic = traffic * pairs_per_launch / tj["pairs_per_launch"]
     out = {
         "metric": METRIC,
         "value": npairs / t64,
@@ -171,22 +275,33 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "twoereduced for all orbital pairs: 81^3 grid, spacing 0.25, 16 hermnorm orbitals "
-                               "(136 unique / 256 ordered pairs), circulant edge %d" % s.circulant_edge,
-                   "grid": n, "orbitals": N, "unique_pairs": npairs, "circulant_edge": s.circulant_edge,
+                               "(136 unique / 256 ordered pairs), circulant edge %d" % M,
+                   "grid": n, "orbitals": N, "unique_pairs": npairs, "circulant_edge": M,
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_" + dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": alg, "avg_launch_ms": avg_ms},
-        "path_hbm_frac": path_bytes / t64 / 1e9 / HBM_PEAK_GBS,
+                     "alg_bytes_per_launch": alg, "pairs_per_launch": pairs_per_launch, "avg_launch_ms": avg_ms},
+        "path_hbm_frac": alg_bytes_per_pair("path", n, M, 16) * npairs / t64 / 1e9 / HBM_PEAK_GBS,
         "stage_ms_per_step_fp64": {k: prof64[k][0] / args.steps for k in stages},
         "mixed": {"value": npairs / t32, "ms_per_step": t32 * 1e3, "speedup_vs_fp64": t64 / t32,
+                  "path_hbm_frac": alg_bytes_per_pair("path", n, M, 8) * npairs / t32 / 1e9 / HBM_PEAK_GBS,
                   "dtype": "f32 butterflies, f64 spectrum construction and reductions",
                   "twoematel_error_vs_fp64": err,
                   "stage_ms_per_step": {k: prof32[k][0] / args.steps for k in stages}},
         "twoematel_ms_fp64": tmat * 1e3,
         "mult_ke": {"orbitals_per_s": N / tke, "ms_per_call_16_orbitals": tke * 1e3},
-        "serving": None,
     }
+    del tm64, tm32, tm, tr
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        print("[bench] batched mixed form at 81^3", file=sys.stderr, flush=True)
+        bm = bench_batched_81(torch, s, phi, npairs, args.steps)
+        bm["speedup_vs_fp64"] = bm["value"] * t64 / npairs
+        out["batched_mixed_81"] = bm
+        print("[bench] 64-set batch", file=sys.stderr, flush=True)
+        out["batch"] = bench_batch(torch, args.quick)
+        print("[bench] serving", file=sys.stderr, flush=True)
+        out["serving"] = bench_serving(torch, args.quick)
     if not args.no_cpu:
         cores = min(os.cpu_count() or 1, 16)
         print(f"[bench] CPU baseline on {cores} threads", file=sys.stderr, flush=True)
