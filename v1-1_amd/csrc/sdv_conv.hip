@@ -65,7 +65,7 @@ struct KArgs {
   z64* out;
   const cplx<T>* tw;
   const T* khat;
-  int lds_tw2, lds_y, lds_z2, lds_in;  // runtime LDS offsets (elements) of the k_yz tables and workspaces
+  int lds_tw2, lds_y;  // runtime LDS offsets (elements) of the k_yz twiddle table and plane
 };
 
 // Everything one work item (one pair's convolution) needs, resolved from the
@@ -81,7 +81,7 @@ struct ItemView {
   z64* outm;           // mirror output column or nullptr
   const cplx<T>* tw;
   const T* khat;
-  int lds_tw2, lds_y, lds_z2, lds_in;
+  int lds_tw2, lds_y;
 };
 
 template <typename T>
@@ -103,8 +103,6 @@ __device__ __forceinline__ ItemView<T> make_view(const KArgs<T>& a, int pidx, in
   v.khat = a.khat;
   v.lds_tw2 = a.lds_tw2;
   v.lds_y = a.lds_y;
-  v.lds_z2 = a.lds_z2;
-  v.lds_in = a.lds_in;
   return v;
 }
 
@@ -114,12 +112,15 @@ struct XCfg {
   static constexpr int LA = 256 / MX;  // x-lines per workgroup
 };
 
+// y/z plane kernel: every y-row and z-column FFT of the plane is owned by one
+// lane group of G = max(P, Q) lanes inside a single wavefront (GPW groups
+// per wave), so the exchanges inside a line FFT need no workgroup barrier.
 template <int P, int Q>
 struct YZCfg {
   static constexpr int R = P * Q;
-  static constexpr int MX = P > Q ? P : Q;
-  static constexpr int H = (R + 1) / 2;                       // rows per z half-pass
-  static constexpr int NT = ((2 * H * MX + 63) / 64) * 64;    // two channel groups of H*MX tasks
+  static constexpr int G = P > Q ? P : Q;
+  static constexpr int GPW = 64 / G;                          // lane groups per wavefront
+  static constexpr int NT = 64 * ((R + GPW - 1) / GPW);       // one group per line (m lines)
 };
 
 // ---------------------------------------------------------------- x forward
@@ -206,207 +207,225 @@ __global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
 // ------------------------------------------------------------- y/z planes
 // One (pair, kx) plane per workgroup.  LDS (elements of cplx<T>):
 //   tw[M]  = w_M^k ;  T2[a*P + b] = w_m^(a b) (a < Q, b < P)
-//   Y      = m rows x pitch : the plane, then its y-transform [ky-pos][z]
-//   Z2     = H rows x pitch : workspace of the odd z channel
-//   IN     = n x n          : (FP32 only) the input plane, kept for both ky channels
-// The z pass runs over two halves of the ky rows; within a half the even and
-// odd z channels run at the same time on two thread groups (even channel in
-// place in Y, odd channel in Z2), so no per-thread state crosses a phase.
-// FP64 has no room for IN: it re-reads the plane for the odd ky channel and
-// parks the even channel's output in the plane's global slot meanwhile.
+//   Y      = m x pitch, the plane: [ky-position][z] in FP64 (a column of one
+//            ky contiguous: the column FFTs, which hold the most registers,
+//            address it from one base with immediate offsets), [z][ky] in FP32
+// Per y channel cy (even, odd):
+//   rows    : y-forward of row z in place (input y < n from HBM), positions
+//             k1*Q + k2 hold frequency k1 + P*k2;
+//   columns : for the column of one ky, both z channels one after the other
+//             by the same lane group (its input kept in registers), each a
+//             forward FFT, the multiply by the real even spectrum and the
+//             inverse, accumulated; the column's m slots are the exchange
+//             scratch of its own FFTs;
+//   rows    : y-inverse of row z, this channel's output part.
+// Five workgroup barriers per plane.  The cy = 0 output part stays in
+// registers (FP32) or is parked in the plane's own HBM slots, which the same
+// lanes have just read for the cy = 1 input (FP64, whose registers are
+// needed for the column FFTs).
 // The LDS offsets are runtime values, so the compiler cannot prove the
-// twiddle tables distinct from the workspaces and does not pin hoisted
-// twiddles in VGPRs across phases.
-// One (pair, kx) plane.  Threads at or beyond NT (a larger block in the
-// multi-set kernel) only take part in the barriers.
+// twiddle tables distinct from the plane and does not pin hoisted twiddles
+// in VGPRs across phases.
+
+// Ordering of LDS accesses between lanes of one wavefront (a wave's LDS
+// operations execute in order; this keeps the compiler from reordering).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // V is KArgs<T> (single set: the fields stay in the kernel-argument segment,
 // where the compiler can reload them instead of holding them in registers) or
 // ItemView<T>; base points at plane number `plane` (of 2m) of this pair.
+// With GUARD (blocks larger than NT) lanes at or beyond NT only take part in
+// the barriers.
 template <typename T, int P, int Q, bool GUARD, typename V>
 __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, unsigned char* smem_raw) {
-  constexpr int R = P * Q, NT = YZCfg<P, Q>::NT, H = YZCfg<P, Q>::H, G = NT / 2, pitch = R | 1, M = 2 * R;
+  constexpr int R = P * Q, M = 2 * R, pitch = R | 1;
+  constexpr int G = YZCfg<P, Q>::G, GPW = YZCfg<P, Q>::GPW, NT = YZCfg<P, Q>::NT;
   constexpr bool KEEP = sizeof(T) == 4;
+  // element (ky position p, row z) at Y[p * SP + z * SR]: FP64 keeps a
+  // column contiguous (its column FFTs are the register-bound phase), FP32
+  // keeps a row contiguous
+  constexpr int SP = sizeof(T) == 8 ? pitch : 1, SR = sizeof(T) == 8 ? 1 : pitch;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* T2 = tw + a.lds_tw2;
   cplx<T>* Y = tw + a.lds_y;
-  cplx<T>* Z2 = tw + a.lds_z2;
-  cplx<T>* IN = tw + a.lds_in;
-  // with GUARD (blocks larger than NT) idle threads get no task and no loop element
-  const int tid = (!GUARD || threadIdx.x < NT) ? (int)threadIdx.x : (1 << 30);
-  const int n = a.n, n2 = a.n2;
+  const int t = (!GUARD || threadIdx.x < NT) ? (int)threadIdx.x : (1 << 29);
+  const int lane = t & 63, gi = lane / G, li = lane - gi * G;
+  const int grp = gi < GPW ? (t >> 6) * GPW + gi : (1 << 29);  // this lane group's row / column
+  const int n = a.n;
   const int cx = plane / R, posx = plane - cx * R;
   const int kx = 2 * (posx / Q + P * (posx % Q)) + cx;
   const int kkx = min(kx, M - kx);
   const T* kh = a.khat + (size_t)kkx * (R + 1) * (R + 1);
-  for (int i = tid; i < M; i += NT) tw[i] = a.tw[i];
-  for (int i = tid; i < R; i += NT) {
+  for (int i = t; i < M; i += NT) tw[i] = a.tw[i];
+  for (int i = t; i < R; i += NT) {
     const int aa = i / P, bb = i - aa * P;
     T2[i] = a.tw[(2 * aa * bb) % M];
   }
-  const int yz = tid % n, yj = tid / n;            // y-line tasks: (z, j2) / (z, k1)
-  const bool ya = yj < Q, yb = yj < P;
-  const int cz = GUARD ? (tid / G >= 1 ? 1 : 0) : tid / G, tg = tid - cz * G;  // z tasks: channel group, index in group
-  const int zrr = tg % H, zj = tg / H;             // (row in half, j2) / (row in half, k1)
-  if (KEEP) {  // the input plane stays in LDS for both ky channels
-    RowCol rc(tid, NT, n);
-    for (int e = tid; e < n2; e += NT, rc.next()) IN[rc.row * n + rc.col] = base[e];
-  }
+  const bool rowok = grp < n, colok = grp < R;
+  cplx<T>* yrow = Y + grp * SR;                // row z = grp: position p at yrow[p * SP]
+  const cplx<T>* grow = base + (size_t)grp * n;  // the same row of the plane in HBM
+  cplx<T> o0[KEEP ? P : 1];                    // cy = 0 output part (FP32)
+  cplx<T> x[P];                                // row input
 
-  cplx<T> acc[P], acc0[KEEP ? P : 1];
-#pragma unroll 1
-  for (int cy = 0; cy < 2; ++cy) {
-    __syncthreads();
-    if (!KEEP) {  // stage the plane, layout [z][y]
-      RowCol rc(tid, NT, n);
-      for (int e = tid; e < n2; e += NT, rc.next()) Y[rc.row * pitch + rc.col] = base[e];
-      __syncthreads();
-    }
-    cplx<T> v[P];
-    if (ya) {
+  // y-forward of row grp for channel cy from x[j1] = plane(z = grp, y = j1 Q + li)
+  auto yfwd = [&](int cy) {
+    if (rowok && li < Q) {
+      cplx<T> v[P];
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
-        const int y = j1 * Q + yj;
-        cplx<T> u = (y < n) ? (KEEP ? IN[yz * n + y] : Y[yz * pitch + y]) : mk((T)0, (T)0);
-        v[j1] = cy ? u * tw[y] : u;
+        const int y = j1 * Q + li;
+        v[j1] = (cy && y < n) ? x[j1] * tw[y] : x[j1];
       }
-    }
-    if (cy == 1) {
-      if (KEEP) {
-#pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) acc0[j1] = acc[j1];
-      } else {  // park the cy = 0 output in the plane's global slot
-        __syncthreads();
-        if (ya) {
-#pragma unroll
-          for (int j1 = 0; j1 < P; ++j1) {
-            const int y = j1 * Q + yj;
-            if (y < n) Y[yz * pitch + y] = acc[j1];
-          }
-        }
-        __syncthreads();
-        RowCol rc(tid, NT, n);
-        for (int e = tid; e < n2; e += NT, rc.next()) base[e] = Y[rc.row * pitch + rc.col];
-      }
-    }
-    __syncthreads();
-    if (ya) {  // y forward, step 1 -> layout [y-position][z]
       DFT<T, P, false>::run(v);
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) Y[(k1 * Q + yj) * pitch + yz] = v[k1] * T2[yj * P + k1];
+      for (int k1 = 0; k1 < P; ++k1) yrow[(k1 * Q + li) * SP] = v[k1] * T2[li * P + k1];
     }
-    __syncthreads();
-    if (yb) {  // y forward, step 2
+    wave_sync();
+    if (rowok && li < P) {
       cplx<T> u[Q];
 #pragma unroll
-      for (int j2 = 0; j2 < Q; ++j2) u[j2] = Y[(yj * Q + j2) * pitch + yz];
+      for (int j2 = 0; j2 < Q; ++j2) u[j2] = yrow[(li * Q + j2) * SP];
       DFT<T, Q, false>::run(u);
 #pragma unroll
-      for (int k2 = 0; k2 < Q; ++k2) Y[(yj * Q + k2) * pitch + yz] = u[k2];
+      for (int k2 = 0; k2 < Q; ++k2) yrow[(li * Q + k2) * SP] = u[k2];
     }
-    // z pass: two halves of the ky rows, both z channels concurrently
-#pragma unroll 1
-    for (int h = 0; h < 2; ++h) {
-      const int r0 = h * H;
-      const int nh = min(H, R - r0);
-      const int r = r0 + zrr;
-      const bool rowok = zrr < nh;
-      cplx<T>* zrow = cz ? Z2 + zrr * pitch : Y + r * pitch;
-      const int k1y = r / Q, k2y = r - (r / Q) * Q;
-      const int ky = 2 * (k1y + P * k2y) + cy;
-      const int kky = min(ky, M - ky);
-      __syncthreads();
-      if (rowok && zj < Q) {  // step 1: both groups read row r of Y, then write their channel
+  };
+  auto load_row = [&]() {
+    if (rowok && li < Q) {
 #pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) {
-          const int z = j1 * Q + zj;
-          const cplx<T> u = (z < n) ? Y[r * pitch + z] : mk((T)0, (T)0);
-          v[j1] = cz ? u * tw[z] : u;
-        }
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int y = j1 * Q + li;
+        x[j1] = y < n ? grow[y] : mk((T)0, (T)0);
       }
-      __syncthreads();
-      if (rowok && zj < Q) {
+    }
+  };
+  // both z channels of column grp (ky position), spectrum of y channel cy
+  auto zpass = [&](int cy) {
+    if (!colok) return;
+    const int c = grp;
+    cplx<T>* ycol = Y + c * SP;  // column c: row z at ycol[z * SR]
+    const int k1y = c / Q, k2y = c - (c / Q) * Q;
+    const int ky = 2 * (k1y + P * k2y) + cy;
+    const int kky = min(ky, M - ky);  // 32-bit offsets from the workgroup-uniform slice base
+    cplx<T> u[P], acc[P];
+    if (li < Q) {
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int z = j1 * Q + li;
+        u[j1] = z < n ? ycol[z * SR] : mk((T)0, (T)0);
+        acc[j1] = mk((T)0, (T)0);
+      }
+    }
+#pragma unroll 1
+    for (int cz = 0; cz < 2; ++cz) {
+      if (li < Q) {
+        cplx<T> v[P];
+#pragma unroll
+        for (int j1 = 0; j1 < P; ++j1) v[j1] = cz ? u[j1] * tw[j1 * Q + li] : u[j1];
         DFT<T, P, false>::run(v);
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) zrow[k1 * Q + zj] = v[k1] * T2[zj * P + k1];
+        for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * T2[li * P + k1];
       }
-      __syncthreads();
-      if (rowok && zj < P) {  // step 2 + spectral multiply + inverse step 1
-        cplx<T> u[Q];
+      wave_sync();
+      if (li < P) {  // step 2, spectral multiply, inverse step 1
+        cplx<T> w[Q];
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) u[j2] = zrow[zj * Q + j2];
-        DFT<T, Q, false>::run(u);
-        const T* khr = kh + kky;
+        for (int j2 = 0; j2 < Q; ++j2) w[j2] = ycol[(li * Q + j2) * SR];
+        DFT<T, Q, false>::run(w);
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) {
-          const int kz = 2 * (zj + P * k2) + cz;
-          const int kkz = min(kz, M - kz);
-          u[k2] = scale(u[k2], khr[(R + 1) * kkz]);
+          const int kz = 2 * (li + P * k2) + cz;
+          w[k2] = scale(w[k2], kh[(R + 1) * min(kz, M - kz) + kky]);
         }
-        DFT<T, Q, true>::run(u);
+        DFT<T, Q, true>::run(w);
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) zrow[zj * Q + j2] = u[j2] * conj(T2[j2 * P + zj]);
+        for (int j2 = 0; j2 < Q; ++j2) ycol[(li * Q + j2) * SR] = w[j2] * conj(T2[j2 * P + li]);
       }
-      __syncthreads();
-      if (rowok && zj < Q) {  // inverse step 2; the odd channel carries w_M^-z
+      wave_sync();
+      if (li < Q) {  // inverse step 2; the odd channel carries w_M^-z
+        cplx<T> v[P];
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) v[k1] = zrow[k1 * Q + zj];
+        for (int k1 = 0; k1 < P; ++k1) v[k1] = ycol[(k1 * Q + li) * SR];
         DFT<T, P, true>::run(v);
 #pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) {
-          const int z = j1 * Q + zj;
-          zrow[z] = cz ? v[j1] * conj(tw[z]) : v[j1];
-        }
-      }
-      __syncthreads();
-      RowCol rc(tid, NT, n);
-      for (int e = tid; e < nh * n; e += NT, rc.next()) {  // even + odd channel
-        cplx<T>* d = Y + (r0 + rc.row) * pitch + rc.col;
-        *d = *d + Z2[rc.row * pitch + rc.col];
+        for (int j1 = 0; j1 < P; ++j1) acc[j1] = acc[j1] + (cz ? v[j1] * conj(tw[j1 * Q + li]) : v[j1]);
       }
     }
-    __syncthreads();
-    if (yb) {  // y inverse, step 1
+    if (li < Q) {
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int z = j1 * Q + li;
+        if (z < n) ycol[z * SR] = acc[j1];
+      }
+    }
+  };
+  // y-inverse of row grp for channel cy -> o[j1] at y = j1 Q + li
+  auto yinv = [&](int cy, cplx<T>* o) {
+    if (rowok && li < P) {
       cplx<T> u[Q];
 #pragma unroll
-      for (int k2 = 0; k2 < Q; ++k2) u[k2] = Y[(yj * Q + k2) * pitch + yz];
+      for (int k2 = 0; k2 < Q; ++k2) u[k2] = yrow[(li * Q + k2) * SP];
       DFT<T, Q, true>::run(u);
 #pragma unroll
-      for (int j2 = 0; j2 < Q; ++j2) Y[(yj * Q + j2) * pitch + yz] = u[j2] * conj(T2[j2 * P + yj]);
+      for (int j2 = 0; j2 < Q; ++j2) yrow[(li * Q + j2) * SP] = u[j2] * conj(T2[j2 * P + li]);
     }
-    __syncthreads();
-    if (ya) {  // y inverse, step 2: this ky channel's output
+    wave_sync();
+    if (rowok && li < Q) {
+      cplx<T> v[P];
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) v[k1] = Y[(k1 * Q + yj) * pitch + yz];
+      for (int k1 = 0; k1 < P; ++k1) v[k1] = yrow[(k1 * Q + li) * SP];
       DFT<T, P, true>::run(v);
 #pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) acc[j1] = cy ? v[j1] * conj(tw[j1 * Q + yj]) : v[j1];
+      for (int j1 = 0; j1 < P; ++j1) o[j1] = cy ? v[j1] * conj(tw[j1 * Q + li]) : v[j1];
     }
-  }
-  // output = cy = 0 part (registers or parked) + cy = 1 part
+  };
+
+  load_row();
+  __syncthreads();  // tables
+  yfwd(0);
   __syncthreads();
-  if (KEEP) {
-    if (ya) {
+  zpass(0);
+  __syncthreads();
+  {
+    cplx<T> o[P];
+    yinv(0, o);
+    load_row();
+    if (KEEP) {
 #pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) {
-        const int y = j1 * Q + yj;
-        if (y < n) Y[yz * pitch + y] = acc0[j1] + acc[j1];
+      for (int j1 = 0; j1 < P; ++j1) o0[j1 % (KEEP ? P : 1)] = o[j1];
+      yfwd(1);
+    } else {
+      yfwd(1);  // consumes x before its HBM slots take the parked cy = 0 output
+      if (rowok && li < Q) {
+#pragma unroll
+        for (int j1 = 0; j1 < P; ++j1) {
+          const int y = j1 * Q + li;
+          if (y < n) base[(size_t)grp * n + y] = o[j1];
+        }
       }
     }
-  } else {
-    RowCol rc(tid, NT, n);
-    for (int e = tid; e < n2; e += NT, rc.next()) Y[rc.row * pitch + rc.col] = base[e];
-    __syncthreads();
-    if (ya) {
+  }
+  __syncthreads();
+  zpass(1);
+  __syncthreads();
+  {
+    cplx<T> o[P];
+    yinv(1, o);
+    if (rowok && li < Q) {
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
-        const int y = j1 * Q + yj;
-        if (y < n) Y[yz * pitch + y] = Y[yz * pitch + y] + acc[j1];
+        const int y = j1 * Q + li;
+        if (y < n) {
+          cplx<T>* d = base + (size_t)grp * n + y;
+          *d = (KEEP ? o0[j1 % (KEEP ? P : 1)] : *d) + o[j1];
+        }
       }
     }
   }
-  __syncthreads();
-  RowCol rc((!GUARD || tid < NT) ? tid : 0, NT, n);
-  for (int e = tid; e < n2; e += NT, rc.next()) base[e] = Y[rc.row * pitch + rc.col];
 }
 
 template <typename T, int P, int Q>
@@ -507,9 +526,7 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   const size_t shX = (size_t)(M + LA * pitch) * sizeof(cplx<T>);
   a.lds_tw2 = M;
   a.lds_y = M + R + ((M + R) & 1);  // keep the workspaces 16-byte aligned for FP32
-  a.lds_z2 = a.lds_y + R * pitch + ((R * pitch) & 1);
-  a.lds_in = a.lds_z2 + YZCfg<P, Q>::H * pitch + ((YZCfg<P, Q>::H * pitch) & 1);
-  const size_t shB = (size_t)(a.lds_in + (sizeof(T) == 4 ? a.n2 : 0)) * sizeof(cplx<T>);
+  const size_t shB = (size_t)(a.lds_y + R * pitch) * sizeof(cplx<T>);
   SDV_REQUIRE(shB <= 160 * 1024, SDV_ERR_UNSUPPORTED, "y/z plane does not fit in LDS");
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
   dim3 gX((a.n2 + LA - 1) / LA, h.npairs);
@@ -559,11 +576,9 @@ __device__ __forceinline__ ItemView<T> batch_view(const BSetDev<T>& s, const BIt
   v.outm = it.mcol >= 0 ? s.out + (size_t)it.mcol * v.n3 : nullptr;
   v.tw = s.tw;
   v.khat = s.khat;
-  const int R = s.m, M = 2 * R, pitch = R | 1, H = (R + 1) / 2;
+  const int R = s.m, M = 2 * R;
   v.lds_tw2 = M;
   v.lds_y = M + R + ((M + R) & 1);
-  v.lds_z2 = v.lds_y + R * pitch + ((R * pitch) & 1);
-  v.lds_in = v.lds_z2 + H * pitch + ((H * pitch) & 1);
   return v;
 }
 
@@ -702,11 +717,11 @@ __global__ void __launch_bounds__(256) k_xinv_grp(const BSetDev<T>* __restrict__
 }
 
 // LDS of the x kernels and of the y/z kernel for half-edge m (elements of
-// cplx<T>; the y/z figure excludes the FP32 input-plane copy of n^2).
+// cplx<T>).
 static size_t lds_x_elems(int m) { return (size_t)2 * m + (size_t)lines_per_block(m) * (m | 1); }
 static size_t lds_yz_elems(int m) {
-  const size_t R = m, M = 2 * R, pitch = R | 1, H = (R + 1) / 2;
-  return (M + R + ((M + R) & 1)) + R * pitch + ((R * pitch) & 1) + H * pitch + ((H * pitch) & 1);
+  const size_t R = m, M = 2 * R, pitch = R | 1;
+  return (M + R + ((M + R) & 1)) + R * pitch;
 }
 
 BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int mode, int grid_mult, int ncu) {
@@ -723,7 +738,6 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
     BWork w;
   };
   std::vector<Cost> wx, wy;
-  std::vector<int> nmax(128, 0);  // largest grid edge per half-edge m (FP32 input-plane copy)
   for (int si = 0; si < (int)sets.size(); ++si) {
     const BatchSetHost& s = sets[si];
     const int n = s.spec->g.n, m = s.spec->g.m;
@@ -733,9 +747,8 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
       bp.grid_x = bp.grid_y = 0;
       return bp;
     }
-    nmax[m] = std::max(nmax[m], n);
     shx = std::max(shx, lds_x_elems(m) * es);
-    shy = std::max(shy, (lds_yz_elems(m) + (precision ? (size_t)n * n : 0)) * es);
+    shy = std::max(shy, lds_yz_elems(m) * es);
     nty = std::max(nty, yz_threads(m));
     for (const PairItem& p : *s.items) {
       BItemDev it;
@@ -773,7 +786,7 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
       while (i < wy.size() && wy[i].m == g.m) ++i;
       g.nwy = (int)i - g.wy0;
       g.shx = lds_x_elems(g.m) * es;
-      g.shy = (lds_yz_elems(g.m) + (precision ? (size_t)nmax[g.m] * nmax[g.m] : 0)) * es;
+      g.shy = lds_yz_elems(g.m) * es;
       bp.groups.push_back(g);
     }
     size_t gi = 0;
