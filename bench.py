@@ -36,6 +36,8 @@ sys.path.insert(0, os.path.join(ROOT, "v1-1_amd"))
 
 METRIC = "pair Coulomb potentials/s at 81^3 (mixed vs FP64), % HBM peak, p99 latency"
 HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_F64 = 78.6    # TFLOP/s dense, MI355X
+MFMA_PEAK_F32 = 157.3
 
 
 def alg_bytes_per_pair(stage, n, M, s_elem):
@@ -243,7 +245,24 @@ def main():
            "twoereduced_max_rel": float((tr - tr64).abs().max() / tr64.abs().max())}
     del tr64
     print("[bench] kinetic apply", file=sys.stderr, flush=True)
-    tke = time_steps(lambda: s.mult_ke(phi), args.steps, 1, torch)
+    ke_res = {}
+    ke_out = {}
+    for prec, name, peak in ((0, "fp64", MFMA_PEAK_F64), (1, "mixed", MFMA_PEAK_F32)):
+        s.set_precision(prec)
+        tk = time_steps(lambda: s.mult_ke(phi), args.steps, 1, torch)
+        ke_out[name] = s.mult_ke(phi)
+        useful = 12.0 * n ** 4 * N                                    # 3 axes x 2 (re, im) x 2 n^4
+        npad, qpad = (n + 3) // 4 * 4, (n + 15) // 16 * 16
+        issued = 3 * 2 * 2.0 * qpad * npad * n * n * N               # padded matrix-core work
+        ke_res[name] = {"ms_per_call_16_orbitals": tk * 1e3, "orbitals_per_s": N / tk,
+                        "mfma_tflops_useful": useful / tk / 1e12, "mfma_tflops_issued": issued / tk / 1e12,
+                        "mfma_peak_tflops": peak, "mfma_utilisation": issued / tk / 1e12 / peak,
+                        "hbm_frac_min_traffic": 8 * 16.0 * n ** 3 * N / tk / 1e9 / HBM_PEAK_GBS}
+    s.set_precision(0)
+    ke_res["mixed"]["error_vs_fp64_max_abs_over_max"] = float(
+        (ke_out["mixed"] - ke_out["fp64"]).abs().max() / ke_out["fp64"].abs().max())
+    del ke_out
+    tke = ke_res["fp64"]["ms_per_call_16_orbitals"] / 1e3
 
     # roofline of the dominant kernel of the FP64 form
     stages = ["xfwd", "yz", "xinv"]
@@ -289,7 +308,8 @@ def main():
                   "twoematel_error_vs_fp64": err,
                   "stage_ms_per_step": {k: prof32[k][0] / args.steps for k in stages}},
         "twoematel_ms_fp64": tmat * 1e3,
-        "mult_ke": {"orbitals_per_s": N / tke, "ms_per_call_16_orbitals": tke * 1e3},
+        "mult_ke": dict(ke_res, note="per-axis contraction on the matrix cores (FP64 v_mfma_f64_16x16x4f64, "
+                                     "mixed v_mfma_f32_16x16x4f32); utilisation = issued matrix flops / peak"),
     }
     del tm64, tm32, tm, tr
     torch.cuda.empty_cache()

@@ -126,10 +126,14 @@ def test_op_tinv_all_circulant_sizes(orc, torch_dev, n):
     assert e32 < 2e-5
 
 
+@pytest.mark.parametrize("n", [41, 21, 7, 81])
 @pytest.mark.parametrize("ecs", [False, True])
-def test_mult_ke(orc, torch_dev, ecs):
+def test_mult_ke(orc, torch_dev, ecs, n):
+    """FP64 matrix-core kinetic apply vs the oracle at 1e-12; the FP32
+    matrix-core (mixed) form within 2e-6 of it.  Edges 7, 21, 41, 81 cover
+    partial q tiles, partial line tiles and the largest grid."""
     from sincdvr import SincDVR, inputs
-    n, dx = 41, 0.3
+    dx = 0.3
     phi = inputs.gaussian_orbitals(n, dx, 3, 4, seed=11)
     sc = inputs.ECS_41 if ecs else None
     say("mult_ke n=%d ecs=%s" % (n, ecs))
@@ -141,6 +145,11 @@ def test_mult_ke(orc, torch_dev, ecs):
         ij, sd = orc.ecs_arrays(n, dx, sc["theta"], sc["stretch"], sc["dist"], sc["smooth"])
     ref = orc.mult_ke(n, ke, phi, ij, sd)
     assert relerr(out, ref) < 1e-12
+    s.set_precision(1)
+    out32 = s.mult_ke(phi)
+    e32 = relerr(out32, ref)
+    say("mult_ke fp32 matrix cores rel err %.3e" % e32)
+    assert e32 < 2e-6
 
 
 def test_mean_field(orc, torch_dev, c0):
