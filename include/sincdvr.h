@@ -116,6 +116,26 @@ int sdv_twoereduced(sdv_ctx* ctx, int lowspf, int highspf, const double* inspfs1
 int sdv_twoereduced_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs10, const void* inspfs20,
                         void* twoereduced, void* stream);
 
+/* One-electron matrix elements (S/matel.f90:241-293 sparseops_matel,
+ * :113-169 pot_matel), the rows after mult_ke in the driver:
+ *   opmatel(numspf, lowspf:highspf)  = inspfs1^H . mult_ke(inspfs2(:, lowspf:highspf))
+ *   potmatel(numspf, lowspf:highspf) = inspfs1^H . (pot * inspfs2(:, lowspf:highspf))
+ * (^H = conjugate transpose for hermnorm, MYGEMM CNORMCHAR 'C'; plain
+ * transpose for cnorm, 'T').  pot is the real nuclear potential (totpoints),
+ * e.g. from sdv_nuclear_potential; the contact, hydrogen-atom and frozen-core
+ * terms of mult_pot/pot_matel are zero for the sinc-DVR project without
+ * frozen orbitals and are not applied.  The kinetic apply runs in the
+ * context's precision.  Grid sums are split over workgroups and reduced in a
+ * fixed order (deterministic). */
+int sdv_sparseops_matel(sdv_ctx* ctx, int lowspf, int highspf, const double* inspfs1, const double* inspfs2,
+                        double* opmatel);
+int sdv_sparseops_matel_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs1, const void* inspfs2,
+                            void* opmatel, void* stream);
+int sdv_pot_matel(sdv_ctx* ctx, int lowspf, int highspf, const double* inspfs1, const double* inspfs2,
+                  const double* pot, double* potmatel);
+int sdv_pot_matel_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs1, const void* inspfs2,
+                      const double* pot, void* potmatel, void* stream);
+
 /* Tuning (no counterpart in the reference, whose circulant edge is fixed at
  * 2*numpoints, SD/coreproject.f90:389-391).  Any circulant edge M = 2m with
  * m >= numpoints gives the same linear convolution; sdv_circulant_sizes lists

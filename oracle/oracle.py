@@ -243,3 +243,20 @@ def nuclear_pot(n, spacing, kernel, centershift, charges, realkind4=False):
     if rc:
         raise ValueError("centershift parity error (SD/sincproject.f90:199-204)")
     return pot
+
+
+def one_matel(n, ke, in1, in2, low=1, high=None, pot=None, cnorm=False, invjac=None, scalediag=None):
+    """S/matel.f90 sparseops_matel / pot_matel: (numspf, low:high) matrix
+    in1^H . op(in2(:, low:high)) with op = mult_ke (pot None) or pot * ;
+    ^T instead of ^H for cnorm.  The kinetic apply is the C restatement;
+    the GEMM is numpy."""
+    in1 = np.asfortranarray(in1, dtype=np.complex128)
+    in2 = np.asfortranarray(in2, dtype=np.complex128)
+    high = in1.shape[1] if high is None else high
+    cols = in2[:, low - 1:high]
+    if pot is None:
+        w = mult_ke(n, ke, cols, invjac, scalediag)
+    else:
+        w = cols * np.asarray(pot, dtype=np.float64)[:, None]
+    a = in1 if cnorm else np.conj(in1)
+    return np.asfortranarray(a.T @ w)

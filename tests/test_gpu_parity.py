@@ -230,3 +230,43 @@ def test_nuclear_potential(orc, torch_dev, realkind4):
         assert relerr(s.nuclear_potential(cs, zc, realkind4=realkind4), ref) < 1e-13
     with pytest.raises(SdvError):
         s.nuclear_potential([[1, 0, 0]], [1.0])
+
+
+@pytest.mark.parametrize("cnorm", [False, True])
+@pytest.mark.parametrize("ecs", [False, True])
+def test_one_electron_matels(orc, torch_dev, cnorm, ecs):
+    """sparseops_matel / pot_matel: FP64 vs the oracle (1e-12), sub-range,
+    bra != ket, device and host entry points; hermnorm matrices Hermitian and
+    cnorm ones symmetric for bra == ket without ECS."""
+    torch = torch_dev
+    from sincdvr import SincDVR, inputs
+    n, dx, N = 21, 0.4, 5
+    phi = inputs.gaussian_orbitals(n, dx, N, 3, 77, cnorm=cnorm)
+    phi2 = np.asfortranarray(phi[:, ::-1] * (0.3 - 0.7j))
+    sc = dict(theta=0.3, stretch=1.0, dist=2.5, smooth=1.5) if ecs else None
+    s = SincDVR(n, dx, N, cnorm=cnorm, scaling=sc)
+    pot = s.nuclear_potential([[0, 0, 0]], [2.0])
+    ke, _, _ = orc.sinedvr(n, dx)
+    ij = sd = None
+    if ecs:
+        ij, sd = orc.ecs_arrays(n, dx, sc["theta"], sc["stretch"], sc["dist"], sc["smooth"])
+    say("one-electron matels cnorm=%s ecs=%s" % (cnorm, ecs))
+    for (lo, hi, p2) in [(1, N, phi), (2, 4, phi2)]:
+        op = s.sparseops_matel(lo, hi, phi, p2)
+        ref = orc.one_matel(n, ke, phi, p2, lo, hi, cnorm=cnorm, invjac=ij, scalediag=sd)
+        assert relerr(op, ref) < 1e-12
+        pm = s.pot_matel(lo, hi, phi, pot, p2)
+        pref = orc.one_matel(n, ke, phi, p2, lo, hi, pot=pot, cnorm=cnorm)
+        assert relerr(pm, pref) < 1e-12
+    t1 = torch.from_numpy(np.ascontiguousarray(phi.T)).cuda()
+    opd = s.sparseops_matel(1, N, t1).cpu().numpy().T
+    pmd = s.pot_matel(1, N, t1, torch.from_numpy(pot).cuda()).cpu().numpy().T
+    op = s.sparseops_matel(1, N, phi)
+    assert np.array_equal(opd, op)
+    assert np.array_equal(pmd, s.pot_matel(1, N, phi, pot))
+    if not ecs:
+        sym = op.T if cnorm else op.conj().T
+        assert relerr(op, sym) < 1e-12
+        pm = s.pot_matel(1, N, phi, pot)
+        assert relerr(pm, pm.T if cnorm else pm.conj().T) < 1e-12
+

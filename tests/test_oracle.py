@@ -315,3 +315,24 @@ def test_nuclear_potential_oracle(orc):
     assert 0 < np.abs(p4 - pot).max() < 1e-6 * np.abs(pot).max()
     with pytest.raises(ValueError):
         orc.nuclear_pot(n, d, kern, [[1, 0, 0]], [1.0])                # odd shift with odd n
+
+
+@pytest.mark.parametrize("cnorm", [False, True])
+def test_one_matel_brute_force(orc, cnorm):
+    """sparseops_matel / pot_matel restatement against explicit sums with the
+    dense n^3 x n^3 kinetic matrix (ECS on), sub-range and bra != ket."""
+    n, d, N = 5, 0.4, 3
+    ke, _, _ = orc.sinedvr(n, d)
+    ij, sd = orc.ecs_arrays(n, d, 0.3, 1.0, 0.3, 0.45)
+    rng = np.random.default_rng(5)
+    a = np.asfortranarray(rng.standard_normal((n ** 3, N)) + 1j * rng.standard_normal((n ** 3, N)))
+    b = np.asfortranarray(rng.standard_normal((n ** 3, N)) + 1j * rng.standard_normal((n ** 3, N)))
+    pot = rng.standard_normal(n ** 3)
+    got = orc.one_matel(n, ke, a, b, 2, 3, cnorm=cnorm, invjac=ij, scalediag=sd)
+    gotp = orc.one_matel(n, ke, a, b, 2, 3, pot=pot, cnorm=cnorm)
+    for jj, col in enumerate((1, 2)):
+        tb = orc.mult_ke_dense(n, ke, b[:, col], ij, sd)
+        for i in range(N):
+            bra = a[:, i] if cnorm else np.conj(a[:, i])
+            assert abs(got[i, jj] - np.sum(bra * tb)) < 1e-12 * np.abs(tb).max() * np.abs(a).sum()
+            assert abs(gotp[i, jj] - np.sum(bra * pot * b[:, col])) < 1e-12 * np.abs(b).sum() * 10

@@ -254,7 +254,8 @@ int sdv_free(sdv_ctx* c) {
   hipFree(c->ke);
   hipFree(c->invj);
   hipFree(c->sdiag);
-  for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2})
+  for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2, &c->ob_ke,
+                          &c->ob_scr, &c->h_pot})
     hipFree(b->p);
   for (auto& kv : c->plans) hipFree(kv.second.dev);
   for (auto& r : c->prof.recs) {
@@ -816,6 +817,41 @@ int sdv_mult_ke_dev(sdv_ctx* c, const void* in, void* out, int howmany, void* st
   return ke_launch(a, c->precision, (hipStream_t)stream, &c->prof);
 }
 
+int sdv_sparseops_matel_dev(sdv_ctx* c, int low, int high, const void* inspfs1, const void* inspfs2, void* opmatel,
+                            void* stream) {
+  SDV_REQUIRE(c && inspfs1 && inspfs2 && opmatel, SDV_ERR_ARG, "null argument");
+  SDV_REQUIRE(low >= 1 && high <= c->N, SDV_ERR_ARG, "spf range outside 1..numspf");
+  if (high < low) return SDV_OK;
+  const int num = high - low + 1;
+  const size_t col = (size_t)c->n3 * sizeof(z64);
+  hipStream_t st = (hipStream_t)stream;
+  SDV_TRY(ensure(c->ob_ke, col * num));
+  SDV_TRY(ensure(c->ob_scr, dot_scratch_bytes(c->n3, c->N, num)));
+  KeArgsHost a;
+  a.n = c->n;
+  a.howmany = num;
+  a.ke = c->ke;
+  a.invj = c->invj;
+  a.sdiag = c->sdiag;
+  a.in = (const char*)inspfs2 + col * (low - 1);
+  a.out = c->ob_ke.p;
+  SDV_TRY(ke_launch(a, c->precision, st, &c->prof));
+  return dot_launch(c->n3, c->N, num, inspfs1, c->ob_ke.p, nullptr, c->cfg.cnormflag ? 0 : 1, c->ob_scr.p, opmatel,
+                    c->N, st);
+}
+
+int sdv_pot_matel_dev(sdv_ctx* c, int low, int high, const void* inspfs1, const void* inspfs2, const double* pot,
+                      void* potmatel, void* stream) {
+  SDV_REQUIRE(c && inspfs1 && inspfs2 && pot && potmatel, SDV_ERR_ARG, "null argument");
+  SDV_REQUIRE(low >= 1 && high <= c->N, SDV_ERR_ARG, "spf range outside 1..numspf");
+  if (high < low) return SDV_OK;
+  const int num = high - low + 1;
+  const size_t col = (size_t)c->n3 * sizeof(z64);
+  SDV_TRY(ensure(c->ob_scr, dot_scratch_bytes(c->n3, c->N, num)));
+  return dot_launch(c->n3, c->N, num, inspfs1, (const char*)inspfs2 + col * (low - 1), pot,
+                    c->cfg.cnormflag ? 0 : 1, c->ob_scr.p, potmatel, c->N, (hipStream_t)stream);
+}
+
 int sdv_mult_reducedpot_dev(sdv_ctx* c, int first, int last, const void* inspfs, void* outspfs,
                             const void* reducedpot, void* stream) {
   SDV_REQUIRE(c && inspfs && outspfs && reducedpot, SDV_ERR_ARG, "null argument");
@@ -897,6 +933,39 @@ int sdv_mult_ke(sdv_ctx* c, const double* in, double* out, int howmany) {
   SDV_TRY(ensure(c->h_out1, bytes));
   SDV_TRY(sdv_mult_ke_dev(c, di, c->h_out1.p, howmany, c->stream));
   SDV_HIP_CHECK(hipMemcpyAsync(out, c->h_out1.p, bytes, hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+int sdv_sparseops_matel(sdv_ctx* c, int low, int high, const double* inspfs1, const double* inspfs2,
+                        double* opmatel) {
+  SDV_REQUIRE(c && inspfs1 && inspfs2 && opmatel, SDV_ERR_ARG, "null argument");
+  if (high < low) return SDV_OK;
+  const size_t pb = (size_t)c->n3 * c->N * sizeof(z64);
+  const void *d1 = nullptr, *d2 = nullptr;
+  SDV_TRY(stage_in(c, c->h_in1, inspfs1, pb, &d1));
+  SDV_TRY(stage_in(c, c->h_in2, inspfs2, pb, &d2));
+  const size_t ob = (size_t)c->N * (high - low + 1) * sizeof(z64);
+  SDV_TRY(ensure(c->h_out2, ob));
+  SDV_TRY(sdv_sparseops_matel_dev(c, low, high, d1, d2, c->h_out2.p, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(opmatel, c->h_out2.p, ob, hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+int sdv_pot_matel(sdv_ctx* c, int low, int high, const double* inspfs1, const double* inspfs2, const double* pot,
+                  double* potmatel) {
+  SDV_REQUIRE(c && inspfs1 && inspfs2 && pot && potmatel, SDV_ERR_ARG, "null argument");
+  if (high < low) return SDV_OK;
+  const size_t pb = (size_t)c->n3 * c->N * sizeof(z64);
+  const void *d1 = nullptr, *d2 = nullptr, *dp = nullptr;
+  SDV_TRY(stage_in(c, c->h_in1, inspfs1, pb, &d1));
+  SDV_TRY(stage_in(c, c->h_in2, inspfs2, pb, &d2));
+  SDV_TRY(stage_in(c, c->h_pot, pot, (size_t)c->n3 * sizeof(double), &dp));
+  const size_t ob = (size_t)c->N * (high - low + 1) * sizeof(z64);
+  SDV_TRY(ensure(c->h_out2, ob));
+  SDV_TRY(sdv_pot_matel_dev(c, low, high, d1, d2, (const double*)dp, c->h_out2.p, c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(potmatel, c->h_out2.p, ob, hipMemcpyDefault, c->stream));
   SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
   return SDV_OK;
 }

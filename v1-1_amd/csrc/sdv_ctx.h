@@ -41,7 +41,7 @@ struct sdv_ctx {
     void* p = nullptr;
     size_t cap = 0;
   };
-  Buf work, scratch, gout, h_in1, h_in2, h_out1, h_out2;
+  Buf work, scratch, gout, h_in1, h_in2, h_out1, h_out2, ob_ke, ob_scr, h_pot;
 };
 
 namespace sdv {

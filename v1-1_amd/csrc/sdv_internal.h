@@ -165,6 +165,12 @@ struct KeArgsHost {
 };
 int ke_launch(const KeArgsHost& a, int precision, hipStream_t st, Prof* prof);
 
+// One-electron matrix elements: out(a + ld b) = sum_x f(A(x, a)) w(x) B(x, b)
+// (f = conj if conj_a; w = 1 if null); scratch of dot_scratch_bytes.
+size_t dot_scratch_bytes(int n3, int na, int nb);
+int dot_launch(int n3, int na, int nb, const void* A, const void* B, const double* w, int conj_a, void* scratch,
+               void* out, int ld, hipStream_t st);
+
 // Mean field
 int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, hipStream_t st);
 int mult_reducedpot_launch(int n3, int N, int nout, const void* inspfs, const void* rp, void* out,

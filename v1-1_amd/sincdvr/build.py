@@ -18,7 +18,7 @@ LIB = os.path.join(PKG, "libsincdvr.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["sdv_abi.hip", "sdv_conv.hip", "sdv_spectrum.hip", "sdv_matel.hip", "sdv_ke.hip",
-           "sdv_meanfield.hip", "sdv_server.hip"]
+           "sdv_meanfield.hip", "sdv_server.hip", "sdv_onebody.hip"]
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result",

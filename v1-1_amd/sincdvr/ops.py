@@ -222,6 +222,44 @@ class SincDVR:
         _lib.check(self._lib.sdv_mult_ke(self._h, _hptr(a), _hptr(out), a.shape[1]))
         return out
 
+    # ------------------------------------------- one-electron matrix elements
+    def sparseops_matel(self, lowspf, highspf, inspfs1, inspfs2=None):
+        """S/matel.f90:241-293: opmatel(numspf, lowspf:highspf) =
+        inspfs1^H . mult_ke(inspfs2(:, lowspf:highspf)) (^T for cnorm)."""
+        return self._one_matel(lowspf, highspf, inspfs1, inspfs2, None)
+
+    def pot_matel(self, lowspf, highspf, inspfs1, pot, inspfs2=None):
+        """S/matel.f90:113-169: potmatel = inspfs1^H . (pot * inspfs2(:, lowspf:highspf))."""
+        return self._one_matel(lowspf, highspf, inspfs1, inspfs2, pot)
+
+    def _one_matel(self, lowspf, highspf, inspfs1, inspfs2, pot):
+        N = self.numspf
+        num = max(highspf - lowspf + 1, 0)
+        same = inspfs2 is None
+        if _is_torch(inspfs1):
+            p1 = inspfs1.contiguous()
+            p2 = p1 if same else inspfs2.contiguous()
+            out = torch.zeros((num, N), dtype=torch.complex128, device=p1.device)
+            if pot is None:
+                _lib.check(self._lib.sdv_sparseops_matel_dev(self._h, lowspf, highspf, _dptr(p1), _dptr(p2),
+                                                             _dptr(out), _stream()))
+            else:
+                pt = pot.contiguous()
+                assert pt.dtype == torch.float64 and pt.is_cuda
+                _lib.check(self._lib.sdv_pot_matel_dev(self._h, lowspf, highspf, _dptr(p1), _dptr(p2),
+                                                       ctypes.c_void_p(pt.data_ptr()), _dptr(out), _stream()))
+            return out
+        p1 = np.asfortranarray(inspfs1, dtype=np.complex128)
+        p2 = p1 if same else np.asfortranarray(inspfs2, dtype=np.complex128)
+        out = np.zeros((N, num), dtype=np.complex128, order="F")
+        if pot is None:
+            _lib.check(self._lib.sdv_sparseops_matel(self._h, lowspf, highspf, _hptr(p1), _hptr(p2), _hptr(out)))
+        else:
+            pt = np.ascontiguousarray(pot, dtype=np.float64)
+            _lib.check(self._lib.sdv_pot_matel(self._h, lowspf, highspf, _hptr(p1), _hptr(p2),
+                                               ctypes.c_void_p(pt.ctypes.data), _hptr(out)))
+        return out
+
     # -------------------------------------------------------- mult_reducedpot
     def mult_reducedpot(self, firstspf, lastspf, inspfs, reducedpot):
         """SD/coreproject.f90:1200-1235."""
