@@ -144,7 +144,8 @@ int sdv_pot_matel_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs1
  * smallest supported m >= numpoints); batches built on the context before the
  * change must be rebuilt.  sdv_set_pair_batch sets how many pair potentials
  * one launch of the convolution pipeline carries (work buffer = pairs x 2m x
- * numpoints^2 complex). */
+ * numpoints^2 complex); 0 (the default) = every pair of the call, up to an
+ * 8 GiB work buffer. */
 int sdv_circulant_sizes(int* half_edges, int cap);
 int sdv_set_circulant(sdv_ctx* ctx, int half_edge);
 int sdv_set_pair_batch(sdv_ctx* ctx, int pairs);

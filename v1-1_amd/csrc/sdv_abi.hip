@@ -299,7 +299,7 @@ int sdv_set_circulant(sdv_ctx* c, int half_edge) {
 }
 
 int sdv_set_pair_batch(sdv_ctx* c, int pairs) {
-  SDV_REQUIRE(c && pairs >= 1, SDV_ERR_ARG, "pair batch must be >= 1");
+  SDV_REQUIRE(c && pairs >= 0, SDV_ERR_ARG, "pair batch must be >= 0 (0: fill the work budget)");
   c->pair_batch = pairs;
   return SDV_OK;
 }
@@ -457,7 +457,12 @@ static int run_pairs(sdv_ctx* c, const IdxPlan& pl, const void* phi1, const void
   const int cn = c->cfg.cnormflag;
   const int np = (int)pl.items.size();
   if (np == 0) return SDV_OK;
-  const int batch = std::min(np, c->pair_batch);
+  int batch = c->pair_batch;
+  if (batch <= 0) {  // every pair of the call in one launch when the budget allows (288 GB of HBM)
+    const size_t per = conv_work_bytes(c->spec->g, 1, c->precision);
+    batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)np, c->work_budget / per));
+  }
+  batch = std::min(np, batch);
   SDV_TRY(ensure(c->work, conv_work_bytes(c->spec->g, batch, c->precision)));
   for (int p0 = 0; p0 < np; p0 += batch) {
     ConvArgsHost a;

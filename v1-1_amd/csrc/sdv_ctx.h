@@ -29,7 +29,8 @@ struct sdv_ctx {
   std::map<std::tuple<int, int, int, int>, sdv::IdxPlan> plans;  // (kind, low, high, sym)
   int n = 0, n3 = 0, N = 0;
   int precision = 0;
-  int pair_batch = 32;
+  int pair_batch = 0;                      // pairs per pipeline launch; 0: as many as the work budget holds
+  size_t work_budget = (size_t)8 << 30;    // bytes of convolution work buffer for pair_batch = 0
   int auto_m = 0;      // half-edge choose_geom picks for n (the default circulant)
   sdv::Spectrum* spec = nullptr;  // shared with every context of the same grid (spectrum cache)
   double* ke = nullptr;

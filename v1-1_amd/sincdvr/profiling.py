@@ -28,7 +28,7 @@ if os.path.dirname(HERE) not in sys.path:
     sys.path.insert(0, os.path.dirname(HERE))
 
 COPY_BYTES = 1 << 30          # read and written once by the calibration copy
-PAIRS_PER_LAUNCH = 32         # the library's default pair batch
+PAIRS_PER_LAUNCH = 136        # every pair of the 16-orbital step in one launch (the default)
 KERNELS = ("k_xfwd", "k_yz", "k_xinv")
 
 
