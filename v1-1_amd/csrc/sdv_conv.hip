@@ -201,7 +201,10 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
 template <typename T, int P, int Q>
 __global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_xfwd<T, P, Q>(make_view(a, blockIdx.y, P * Q), blockIdx.x, smem_raw);
+  // pairs run fastest across the grid: the workgroups resident at one time
+  // work on the same line chunk of many pairs, so each orbital chunk is
+  // fetched from HBM once and served to its other pairs from L2
+  body_xfwd<T, P, Q>(make_view(a, blockIdx.x, P * Q), blockIdx.y, smem_raw);
 }
 
 // ------------------------------------------------------------- y/z planes
@@ -532,7 +535,7 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   dim3 gX((a.n2 + LA - 1) / LA, h.npairs);
   dim3 gB(2 * R, h.npairs);
   hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xfwd<T, P, Q>), gX, dim3(256), shX, st, a);
+  hipLaunchKernelGGL((k_xfwd<T, P, Q>), dim3(h.npairs, gX.x), dim3(256), shX, st, a);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
@@ -774,7 +777,15 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
     if (a.w.item != b.w.item) return a.w.item < b.w.item;
     return a.w.sub < b.w.sub;
   };
-  std::stable_sort(wx.begin(), wx.end(), order);
+  // line chunks: chunk-major within a size, so the workgroups resident at
+  // one time read the same orbital chunks (L2 reuse across pairs)
+  auto order_x = [mode](const Cost& a, const Cost& b) {
+    if (mode == BATCH_GROUPED && a.m != b.m) return a.m > b.m;
+    if (a.c != b.c) return a.c > b.c;
+    if (a.w.sub != b.w.sub) return a.w.sub < b.w.sub;
+    return a.w.item < b.w.item;
+  };
+  std::stable_sort(wx.begin(), wx.end(), order_x);
   std::stable_sort(wy.begin(), wy.end(), order);
   for (auto& c : wx) bp.wx.push_back(c.w);
   for (auto& c : wy) bp.wy.push_back(c.w);
