@@ -115,12 +115,18 @@ struct XCfg {
 // y/z plane kernel: every y-row and z-column FFT of the plane is owned by one
 // lane group of G = max(P, Q) lanes inside a single wavefront (GPW groups
 // per wave), so the exchanges inside a line FFT need no workgroup barrier.
-template <int P, int Q>
+// A group could take LPG lines one after the other (smaller workgroups, two
+// per CU for FP32's 54 KB of LDS at m = 81); measured at 81^3 that is 1.4x
+// slower in FP32 than one line per group at 12 waves, so LPG = 1.
+template <typename T, int P, int Q>
 struct YZCfg {
   static constexpr int R = P * Q;
   static constexpr int G = P > Q ? P : Q;
-  static constexpr int GPW = 64 / G;                          // lane groups per wavefront
-  static constexpr int NT = 64 * ((R + GPW - 1) / GPW);       // one group per line (m lines)
+  static constexpr int GPW = 64 / G;                                   // lane groups per wavefront
+  static constexpr int LPG = 1;                                        // lines per group
+  static constexpr int NG = (R + LPG - 1) / LPG;                       // groups needed
+  static constexpr int NT = 64 * ((NG + GPW - 1) / GPW);
+  static constexpr int NGB = NT / 64 * GPW;                            // groups in the workgroup
 };
 
 // ---------------------------------------------------------------- x forward
@@ -246,7 +252,8 @@ __device__ __forceinline__ void wave_sync() {
 template <typename T, int P, int Q, bool GUARD, typename V>
 __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, unsigned char* smem_raw) {
   constexpr int R = P * Q, M = 2 * R, pitch = R | 1;
-  constexpr int G = YZCfg<P, Q>::G, GPW = YZCfg<P, Q>::GPW, NT = YZCfg<P, Q>::NT;
+  typedef YZCfg<T, P, Q> C;
+  constexpr int G = C::G, GPW = C::GPW, NT = C::NT, LPG = C::LPG, NGB = C::NGB;
   constexpr bool KEEP = sizeof(T) == 4;
   // element (ky position p, row z) at Y[p * SP + z * SR]: FP64 keeps a
   // column contiguous (its column FFTs are the register-bound phase), FP32
@@ -257,7 +264,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   cplx<T>* Y = tw + a.lds_y;
   const int t = (!GUARD || threadIdx.x < NT) ? (int)threadIdx.x : (1 << 29);
   const int lane = t & 63, gi = lane / G, li = lane - gi * G;
-  const int grp = gi < GPW ? (t >> 6) * GPW + gi : (1 << 29);  // this lane group's row / column
+  const int grp = gi < GPW ? (t >> 6) * GPW + gi : (1 << 28);  // lines grp, grp + NGB, ...
   const int n = a.n;
   const int cx = plane / R, posx = plane - cx * R;
   const int kx = 2 * (posx / Q + P * (posx % Q)) + cx;
@@ -268,14 +275,23 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     const int aa = i / P, bb = i - aa * P;
     T2[i] = a.tw[(2 * aa * bb) % M];
   }
-  const bool rowok = grp < n, colok = grp < R;
-  cplx<T>* yrow = Y + grp * SR;                // row z = grp: position p at yrow[p * SP]
-  const cplx<T>* grow = base + (size_t)grp * n;  // the same row of the plane in HBM
-  cplx<T> o0[KEEP ? P : 1];                    // cy = 0 output part (FP32)
-  cplx<T> x[P];                                // row input
+  cplx<T> o0[KEEP ? LPG : 1][P];  // cy = 0 output part (FP32)
 
-  // y-forward of row grp for channel cy from x[j1] = plane(z = grp, y = j1 Q + li)
-  auto yfwd = [&](int cy) {
+  // row input x[j1] = plane(z = g, y = j1 Q + li)
+  auto load_row = [&](int g, cplx<T>* x) {
+    if (g < n && li < Q) {
+      const cplx<T>* grow = base + (size_t)g * n;
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) {
+        const int y = j1 * Q + li;
+        x[j1] = y < n ? grow[y] : mk((T)0, (T)0);
+      }
+    }
+  };
+  // y-forward of row g for channel cy (row z = g: position p at yrow[p * SP])
+  auto yfwd = [&](int cy, int g, const cplx<T>* x) {
+    const bool rowok = g < n;
+    cplx<T>* yrow = Y + g * SR;
     if (rowok && li < Q) {
       cplx<T> v[P];
 #pragma unroll
@@ -297,19 +313,9 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       for (int k2 = 0; k2 < Q; ++k2) yrow[(li * Q + k2) * SP] = u[k2];
     }
   };
-  auto load_row = [&]() {
-    if (rowok && li < Q) {
-#pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) {
-        const int y = j1 * Q + li;
-        x[j1] = y < n ? grow[y] : mk((T)0, (T)0);
-      }
-    }
-  };
-  // both z channels of column grp (ky position), spectrum of y channel cy
-  auto zpass = [&](int cy) {
-    if (!colok) return;
-    const int c = grp;
+  // both z channels of column c (ky position), spectrum of y channel cy
+  auto zpass = [&](int cy, int c) {
+    if (c >= R) return;
     cplx<T>* ycol = Y + c * SP;  // column c: row z at ycol[z * SR]
     const int k1y = c / Q, k2y = c - (c / Q) * Q;
     const int ky = 2 * (k1y + P * k2y) + cy;
@@ -366,8 +372,10 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       }
     }
   };
-  // y-inverse of row grp for channel cy -> o[j1] at y = j1 Q + li
-  auto yinv = [&](int cy, cplx<T>* o) {
+  // y-inverse of row g for channel cy -> o[j1] at y = j1 Q + li
+  auto yinv = [&](int cy, int g, cplx<T>* o) {
+    const bool rowok = g < n;
+    cplx<T>* yrow = Y + g * SR;
     if (rowok && li < P) {
       cplx<T> u[Q];
 #pragma unroll
@@ -387,44 +395,54 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     }
   };
 
-  load_row();
   __syncthreads();  // tables
-  yfwd(0);
+#pragma unroll
+  for (int r = 0; r < LPG; ++r) {
+    cplx<T> x[P];
+    load_row(grp + r * NGB, x);
+    yfwd(0, grp + r * NGB, x);
+  }
   __syncthreads();
-  zpass(0);
+#pragma unroll
+  for (int r = 0; r < LPG; ++r) zpass(0, grp + r * NGB);
   __syncthreads();
-  {
-    cplx<T> o[P];
-    yinv(0, o);
-    load_row();
+#pragma unroll
+  for (int r = 0; r < LPG; ++r) {
+    const int g = grp + r * NGB;
+    cplx<T> o[P], x[P];
+    yinv(0, g, o);
+    load_row(g, x);
     if (KEEP) {
 #pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) o0[j1 % (KEEP ? P : 1)] = o[j1];
-      yfwd(1);
+      for (int j1 = 0; j1 < P; ++j1) o0[KEEP ? r : 0][j1] = o[j1];
+      yfwd(1, g, x);
     } else {
-      yfwd(1);  // consumes x before its HBM slots take the parked cy = 0 output
-      if (rowok && li < Q) {
+      yfwd(1, g, x);  // consumes x before its HBM slots take the parked cy = 0 output
+      if (g < n && li < Q) {
 #pragma unroll
         for (int j1 = 0; j1 < P; ++j1) {
           const int y = j1 * Q + li;
-          if (y < n) base[(size_t)grp * n + y] = o[j1];
+          if (y < n) base[(size_t)g * n + y] = o[j1];
         }
       }
     }
   }
   __syncthreads();
-  zpass(1);
+#pragma unroll
+  for (int r = 0; r < LPG; ++r) zpass(1, grp + r * NGB);
   __syncthreads();
-  {
+#pragma unroll
+  for (int r = 0; r < LPG; ++r) {
+    const int g = grp + r * NGB;
     cplx<T> o[P];
-    yinv(1, o);
-    if (rowok && li < Q) {
+    yinv(1, g, o);
+    if (g < n && li < Q) {
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
         const int y = j1 * Q + li;
         if (y < n) {
-          cplx<T>* d = base + (size_t)grp * n + y;
-          *d = (KEEP ? o0[j1 % (KEEP ? P : 1)] : *d) + o[j1];
+          cplx<T>* d = base + (size_t)g * n + y;
+          *d = (KEEP ? o0[KEEP ? r : 0][j1] : *d) + o[j1];
         }
       }
     }
@@ -432,7 +450,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 }
 
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz(KArgs<T> a) {
+__global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   body_yz<T, P, Q, false>(a, a.X + ((size_t)blockIdx.y * 2 * (P * Q) + blockIdx.x) * a.n2, blockIdx.x, smem_raw);
 }
@@ -539,7 +557,7 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_yz<T, P, Q>), gB, dim3(YZCfg<P, Q>::NT), shB, st, a);
+  hipLaunchKernelGGL((k_yz<T, P, Q>), gB, dim3(YZCfg<T, P, Q>::NT), shB, st, a);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
@@ -679,8 +697,8 @@ static int lines_per_block(int m) {
 #undef SDV_LA
   return 0;
 }
-static int yz_threads(int m) {
-#define SDV_NT(P_, Q_) return YZCfg<P_, Q_>::NT
+static int yz_threads(int m, int precision) {
+#define SDV_NT(P_, Q_) return precision ? YZCfg<float, P_, Q_>::NT : YZCfg<double, P_, Q_>::NT
   SDV_SIZE_SWITCH(m, SDV_NT)
 #undef SDV_NT
   return 0;
@@ -699,7 +717,7 @@ __global__ void __launch_bounds__(256) k_xfwd_grp(const BSetDev<T>* __restrict__
 }
 
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__((YZCfg<P, Q>::NT)) k_yz_grp(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz_grp(const BSetDev<T>* __restrict__ sets,
                                                                const BItemDev* __restrict__ items,
                                                                const BWork* __restrict__ work, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -745,14 +763,14 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
     const BatchSetHost& s = sets[si];
     const int n = s.spec->g.n, m = s.spec->g.m;
     const int LA = lines_per_block(m);
-    if (LA <= 0 || yz_threads(m) <= 0 || m >= 128) {  // every context's m comes from choose_geom
+    if (LA <= 0 || yz_threads(m, precision) <= 0 || m >= 128) {  // every context's m comes from choose_geom
       bp.items.clear();
       bp.grid_x = bp.grid_y = 0;
       return bp;
     }
     shx = std::max(shx, lds_x_elems(m) * es);
     shy = std::max(shy, lds_yz_elems(m) * es);
-    nty = std::max(nty, yz_threads(m));
+    nty = std::max(nty, yz_threads(m, precision));
     for (const PairItem& p : *s.items) {
       BItemDev it;
       it.set = si;
@@ -864,7 +882,7 @@ static int launch_group(const BatchGroup& g, const BatchDev& d, hipStream_t st, 
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_yz_grp<T, P, Q>), dim3(g.nwy), dim3(YZCfg<P, Q>::NT), g.shy, st, sets, d.items,
+  hipLaunchKernelGGL((k_yz_grp<T, P, Q>), dim3(g.nwy), dim3(YZCfg<T, P, Q>::NT), g.shy, st, sets, d.items,
                      d.wy + g.wy0, X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
