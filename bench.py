@@ -64,7 +64,7 @@ def time_steps(fn, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(n, dx, phi, cores, nbatch=8):
+def cpu_baseline(n, dx, phi, cores, nbatch=16):
     """The oracle (a C port of the reference's CPU path) on `nbatch` spf2b
     batches of call_twoe_matel00 -- each N densities phi(:,a)*conj(phi(:,b))
     convolved by 3 x (1-D FFT + transpose) at M = 2n = 162 with the kernel FFT
