@@ -54,6 +54,8 @@ struct RowCol {
   }
 };
 
+constexpr int kXThreads = 512;
+
 template <typename T>
 struct KArgs {
   int n, n2, n3, npairs;
@@ -109,7 +111,8 @@ __device__ __forceinline__ ItemView<T> make_view(const KArgs<T>& a, int pidx, in
 template <int P, int Q>
 struct XCfg {
   static constexpr int MX = P > Q ? P : Q;
-  static constexpr int LA = 256 / MX;  // x-lines per workgroup
+  static constexpr int XT = kXThreads;  // workgroup size of the x kernels
+  static constexpr int LA = XT / MX;    // x-lines per workgroup (56 at m = 81: 0.9 KB runs per plane row)
 };
 
 // y/z plane kernel: every y-row and z-column FFT of the plane is owned by one
@@ -146,7 +149,7 @@ __device__ __forceinline__ void xfwd_step2(const ItemView<T>& a, const cplx<T>* 
   }
 }
 
-// x forward of lines [chunk*LA, chunk*LA + LA) of one pair (256 threads).
+// x forward of lines [chunk*LA, chunk*LA + LA) of one pair (kXThreads threads).
 template <typename T, int P, int Q>
 __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsigned char* smem_raw) {
   constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
@@ -156,11 +159,11 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
   const int n = a.n;
   const int l0 = chunk * LA;
   const int nl = min(LA, a.n2 - l0);
-  for (int i = tid; i < M; i += 256) tw[i] = a.tw[i];
+  for (int i = tid; i < M; i += XCfg<P, Q>::XT) tw[i] = a.tw[i];
   const z64* pa = a.pa + (size_t)l0 * n;
   const z64* pb = a.pb + (size_t)l0 * n;
-  RowCol rc(tid, 256, n);
-  for (int e = tid; e < nl * n; e += 256, rc.next()) {
+  RowCol rc(tid, XCfg<P, Q>::XT, n);
+  for (int e = tid; e < nl * n; e += XCfg<P, Q>::XT, rc.next()) {
     z64 v;
     if (a.density_mode) {
       v = pa[e];
@@ -205,7 +208,7 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
 }
 
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(256) k_xfwd(KArgs<T> a) {
+__global__ void __launch_bounds__(kXThreads) k_xfwd(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // pairs run fastest across the grid: the workgroups resident at one time
   // work on the same line chunk of many pairs, so each orbital chunk is
@@ -457,7 +460,7 @@ __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz(KArgs<T> a) {
 
 // ---------------------------------------------------------------- x inverse
 // x inverse of lines [chunk*LA, chunk*LA + LA) of one pair and the output
-// scatter (256 threads).
+// scatter (kXThreads threads).
 template <typename T, int P, int Q>
 __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsigned char* smem_raw) {
   constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
@@ -467,15 +470,15 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
   const int n = a.n;
   const int l0 = chunk * LA;
   const int nl = min(LA, a.n2 - l0);
-  for (int i = tid; i < M; i += 256) tw[i] = a.tw[i];
+  for (int i = tid; i < M; i += XCfg<P, Q>::XT) tw[i] = a.tw[i];
   const int line = tid % LA, jj = tid / LA;
   cplx<T> acc[P];
 #pragma unroll
   for (int j = 0; j < P; ++j) acc[j] = mk((T)0, (T)0);
   for (int c = 0; c < 2; ++c) {
     const cplx<T>* src = a.X + (size_t)c * R * a.n2 + l0;
-    RowCol rc(tid, 256, nl);
-    for (int e = tid; e < R * nl; e += 256, rc.next()) buf[rc.col * pitch + rc.row] = src[(size_t)rc.row * a.n2 + rc.col];
+    RowCol rc(tid, XCfg<P, Q>::XT, nl);
+    for (int e = tid; e < R * nl; e += XCfg<P, Q>::XT, rc.next()) buf[rc.col * pitch + rc.row] = src[(size_t)rc.row * a.n2 + rc.col];
     __syncthreads();
     if (jj < P && line < nl) {  // inverse step 1: task (line, k1)
       cplx<T> u[Q];
@@ -507,8 +510,8 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
   // twoereduced(:, col) and the mirror column (SD/coreproject.f90:521,528)
   z64* o = a.out + (size_t)l0 * n;
   z64* om = a.outm ? a.outm + (size_t)l0 * n : nullptr;
-  RowCol rc(tid, 256, n);
-  for (int e = tid; e < nl * n; e += 256, rc.next()) {
+  RowCol rc(tid, XCfg<P, Q>::XT, n);
+  for (int e = tid; e < nl * n; e += XCfg<P, Q>::XT, rc.next()) {
     const cplx<T> v = buf[rc.row * pitch + rc.col];
     z64 d = mk((double)v.re, (double)v.im);
     o[e] = d;
@@ -520,7 +523,7 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
 }
 
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(256) k_xinv(KArgs<T> a) {
+__global__ void __launch_bounds__(kXThreads) k_xinv(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   body_xinv<T, P, Q>(make_view(a, blockIdx.y, P * Q), blockIdx.x, smem_raw);
 }
@@ -550,10 +553,13 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   const size_t shB = (size_t)(a.lds_y + R * pitch) * sizeof(cplx<T>);
   SDV_REQUIRE(shB <= 160 * 1024, SDV_ERR_UNSUPPORTED, "y/z plane does not fit in LDS");
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
+  SDV_REQUIRE(shX <= 160 * 1024, SDV_ERR_UNSUPPORTED, "x lines do not fit in LDS");
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xfwd<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shX));
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xinv<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shX));
   dim3 gX((a.n2 + LA - 1) / LA, h.npairs);
   dim3 gB(2 * R, h.npairs);
   hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xfwd<T, P, Q>), dim3(h.npairs, gX.x), dim3(256), shX, st, a);
+  hipLaunchKernelGGL((k_xfwd<T, P, Q>), dim3(h.npairs, gX.x), dim3(kXThreads), shX, st, a);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
@@ -561,7 +567,7 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xinv<T, P, Q>), gX, dim3(256), shX, st, a);
+  hipLaunchKernelGGL((k_xinv<T, P, Q>), gX, dim3(kXThreads), shX, st, a);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XINV, e0, st);
   return SDV_OK;
@@ -640,7 +646,7 @@ __device__ __noinline__ void call_xinv(const ItemView<T>* v, int sub, unsigned c
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_xfwd_multi(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__(kXThreads) k_xfwd_multi(const BSetDev<T>* __restrict__ sets,
                                                      const BItemDev* __restrict__ items,
                                                      const BWork* __restrict__ work, int nwork, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -674,7 +680,7 @@ __global__ void __launch_bounds__(768) k_yz_multi(const BSetDev<T>* __restrict__
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_xinv_multi(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__(kXThreads) k_xinv_multi(const BSetDev<T>* __restrict__ sets,
                                                      const BItemDev* __restrict__ items,
                                                      const BWork* __restrict__ work, int nwork, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -707,7 +713,7 @@ static int yz_threads(int m, int precision) {
 // Grouped schedule: one work entry per workgroup, block shape and LDS of
 // this circulant size (as the single-set kernels).
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(256) k_xfwd_grp(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__(kXThreads) k_xfwd_grp(const BSetDev<T>* __restrict__ sets,
                                                    const BItemDev* __restrict__ items,
                                                    const BWork* __restrict__ work, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -728,7 +734,7 @@ __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz_grp(const BSetDev<T
 }
 
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(256) k_xinv_grp(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__(kXThreads) k_xinv_grp(const BSetDev<T>* __restrict__ sets,
                                                    const BItemDev* __restrict__ items,
                                                    const BWork* __restrict__ work, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -853,7 +859,7 @@ static int batch_launch_fused(const BatchPlan& bp, const BatchDev& d, hipStream_
                                     (int)bp.shx));
   const BSetDev<T>* sets = (const BSetDev<T>*)d.sets;
   hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL(k_xfwd_multi<T>, dim3(bp.grid_x), dim3(256), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
+  hipLaunchKernelGGL(k_xfwd_multi<T>, dim3(bp.grid_x), dim3(kXThreads), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
                      (cplx<T>*)d.X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
@@ -863,7 +869,7 @@ static int batch_launch_fused(const BatchPlan& bp, const BatchDev& d, hipStream_
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL(k_xinv_multi<T>, dim3(bp.grid_x), dim3(256), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
+  hipLaunchKernelGGL(k_xinv_multi<T>, dim3(bp.grid_x), dim3(kXThreads), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
                      (cplx<T>*)d.X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XINV, e0, st);
@@ -875,10 +881,14 @@ static int launch_group(const BatchGroup& g, const BatchDev& d, hipStream_t st, 
   SDV_REQUIRE(g.shy <= 160 * 1024, SDV_ERR_UNSUPPORTED, "batch plane does not fit in LDS");
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz_grp<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)g.shy));
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xfwd_grp<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)g.shx));
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xinv_grp<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)g.shx));
   const BSetDev<T>* sets = (const BSetDev<T>*)d.sets;
   cplx<T>* X = (cplx<T>*)d.X;
   hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xfwd_grp<T, P, Q>), dim3(g.nwx), dim3(256), g.shx, st, sets, d.items, d.wx + g.wx0, X);
+  hipLaunchKernelGGL((k_xfwd_grp<T, P, Q>), dim3(g.nwx), dim3(kXThreads), g.shx, st, sets, d.items, d.wx + g.wx0, X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
@@ -887,7 +897,7 @@ static int launch_group(const BatchGroup& g, const BatchDev& d, hipStream_t st, 
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xinv_grp<T, P, Q>), dim3(g.nwx), dim3(256), g.shx, st, sets, d.items, d.wx + g.wx0, X);
+  hipLaunchKernelGGL((k_xinv_grp<T, P, Q>), dim3(g.nwx), dim3(kXThreads), g.shx, st, sets, d.items, d.wx + g.wx0, X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XINV, e0, st);
   return SDV_OK;
