@@ -18,7 +18,7 @@
 
 namespace sdv {
 
-static std::string g_err;
+static thread_local std::string g_err;  // per host thread, like errno
 void set_error(const std::string& msg) { g_err = msg; }
 const char* last_error() { return g_err.c_str(); }
 

@@ -263,7 +263,14 @@ int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hip
   const long ns3 = (long)NS * NS * NS;
   const long n3 = (long)n * n * n;
   const int m1 = g.m + 1;
-  double *TR = nullptr, *T2 = nullptr, *W = nullptr, *E = nullptr, *S1 = nullptr, *S2 = nullptr, *S3 = nullptr;
+  // temporaries of the construction, released on every exit path
+  struct Tmp {
+    double *TR = nullptr, *T2 = nullptr, *W = nullptr, *E = nullptr, *S1 = nullptr, *S2 = nullptr, *S3 = nullptr;
+    ~Tmp() {
+      for (double* p : {TR, T2, W, E, S1, S2, S3}) (void)hipFree(p);
+    }
+  } t;
+  double *&TR = t.TR, *&T2 = t.T2, *&W = t.W, *&E = t.E, *&S1 = t.S1, *&S2 = t.S2, *&S3 = t.S3;
   SDV_HIP_CHECK(hipMalloc(&TR, ns3 * sizeof(double)));
   SDV_HIP_CHECK(hipMalloc(&T2, ns3 * sizeof(double)));
   SDV_HIP_CHECK(hipMalloc(&W, (size_t)NS * NS * sizeof(double)));
@@ -302,13 +309,6 @@ int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hip
   hipLaunchKernelGGL(k_twiddles, dim3(nblk(g.M, 256)), dim3(256), 0, st, (z64*)s->tw64, (c32*)s->tw32, g.M);
   SDV_HIP_CHECK(hipGetLastError());
   SDV_HIP_CHECK(hipStreamSynchronize(st));
-  hipFree(TR);
-  hipFree(T2);
-  hipFree(W);
-  hipFree(E);
-  hipFree(S1);
-  hipFree(S2);
-  hipFree(S3);
   return SDV_OK;
 }
 
