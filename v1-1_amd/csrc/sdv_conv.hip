@@ -322,7 +322,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     cplx<T>* ycol = Y + c * SP;  // column c: row z at ycol[z * SR]
     const int k1y = c / Q, k2y = c - (c / Q) * Q;
     const int ky = 2 * (k1y + P * k2y) + cy;
-    const int kky = min(ky, M - ky);  // 32-bit offsets from the workgroup-uniform slice base
+    const int kcol = (R + 1) * min(ky, M - ky);  // this column's z frequencies, adjacent; 32-bit offset from a uniform base
     cplx<T> u[P], acc[P];
     if (li < Q) {
 #pragma unroll
@@ -351,7 +351,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) {
           const int kz = 2 * (li + P * k2) + cz;
-          w[k2] = scale(w[k2], kh[(R + 1) * min(kz, M - kz) + kky]);
+          w[k2] = scale(w[k2], kh[kcol + min(kz, M - kz)]);
         }
         DFT<T, Q, true>::run(w);
 #pragma unroll

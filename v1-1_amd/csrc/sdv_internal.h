@@ -52,7 +52,7 @@ struct Spectrum {
   Geom g;
   void* tw64 = nullptr;     // M x z64   exp(-2 pi i k / M)
   void* tw32 = nullptr;     // M x c32
-  double* khat64 = nullptr; // (m+1)^3, [kkx][kkz][kky], includes 1/M^3
+  double* khat64 = nullptr; // (m+1)^3, [kkx][kky][kkz], includes 1/M^3
   float* khat32 = nullptr;
   double* koct = nullptr;   // n^3 octant of the real-space kernel K(i,j,k), i,j,k >= 0
 };

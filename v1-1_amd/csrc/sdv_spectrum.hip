@@ -199,7 +199,7 @@ __global__ void k_khat_permute(const double* S, double* kh64, float* kh32, int m
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)m1 * m1 * m1) return;
   const int kx = idx % m1, ky = (idx / m1) % m1, kz = idx / ((long)m1 * m1);
-  const long o = ky + (long)m1 * (kz + (long)m1 * kx);
+  const long o = kz + (long)m1 * (ky + (long)m1 * kx);  // [kkx][kky][kkz]: the z frequencies of one column adjacent
   kh64[o] = S[idx];
   kh32[o] = (float)S[idx];
 }
