@@ -50,6 +50,9 @@ typedef struct sdv_config {
   int notwoflag;          /* call_twoe_matel00 returns zeros (SD/coreproject.f90:1021)  */
   int debugflag;          /* 4040/4242/4343: op_tinv returns zeros (:260-263)           */
   int precision;          /* 0: FP64 plain drop-in; 1: FP32 butterflies (mixed form)    */
+  int maskflag;           /* 1 = kinetic apply through the edge mask when scalingflag = 0
+                             (SD/coreproject.f90:1355-1376, sinc_params.f90:73)          */
+  int masknumpoints;      /* mask width in points per end (SD/init_sinc_new.f90:580-599) */
 } sdv_config;
 
 /* Fill `cfg` with the reference defaults (SD/sinc_params.f90). */

@@ -202,6 +202,23 @@ def mult_ke(n, ke, inp, invjac=None, scalediag=None):
     return out
 
 
+def mask_function(n, masknumpoints):
+    m = np.zeros(n)
+    lib().orc_mask_function(ctypes.c_int(n), ctypes.c_int(masknumpoints), _p(m))
+    return m
+
+
+def mult_ke_mask(n, ke, mask, inp):
+    """mult_ke with maskflag on (scalingflag 0)."""
+    inp = np.asfortranarray(inp, dtype=np.complex128)
+    if inp.ndim == 1:
+        inp = inp.reshape(-1, 1, order="F")
+    out = np.zeros_like(inp, order="F")
+    lib().orc_mult_ke_mask(ctypes.c_int(n), _p(np.ascontiguousarray(ke)), _p(np.ascontiguousarray(mask)),
+                           _p(inp), _p(out), ctypes.c_int(inp.shape[1]))
+    return out
+
+
 def mult_ke_dense(n, ke, inp, invjac=None, scalediag=None):
     inp = np.ascontiguousarray(inp, dtype=np.complex128).ravel()
     out = np.zeros_like(inp)

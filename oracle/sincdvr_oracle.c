@@ -27,6 +27,9 @@
  *   orc_mult_reducedpot, orc_reducedpot   numpy einsum / matmul
  */
 #include <math.h>
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -630,6 +633,63 @@ void orc_mult_ke(int n, const double *ke, const double *invjac_, const double *s
   if (sd)
     for (int h = 0; h < howmany; h++)
       for (size_t p = 0; p < n3; p++) out[h * n3 + p] = cadd(out[h * n3 + p], cmul(in[h * n3 + p], sd[p]));   /* :1382-1388 */
+  free(temp); free(temp2);
+}
+
+/* maskfunction of init_project (SD/init_sinc_new.f90:580-599): 1, times
+ * (1 + cos(pi ii/(mnp+1)))/2 at the ii-th point in from each end
+ * (ii = 1..mnp; overlapping ends multiply twice). */
+void orc_mask_function(int n, int mnp, double *m) {
+  for (int i = 0; i < n; i++) m[i] = 1.0;
+  for (int ii = 1; ii <= mnp; ii++) {
+    const double rsum = (1.0 + cos(M_PI * ii / (mnp + 1))) / 2.0;
+    m[(mnp + 1 - ii) - 1] *= rsum;
+    m[(n - mnp + ii) - 1] *= rsum;
+  }
+}
+
+/* mult_ke with maskflag (scalingflag = 0), SD/coreproject.f90:1355-1376 and
+ * divide_mask / mult_mask (:1273-1319): for each axis d
+ *   temp = ((in / m(x)) / m(y)) / m(z);  out += ((K_d temp * m(x)) * m(y)) * m(z)
+ * with the same mask on the three axes (cubic grid). */
+void orc_mult_ke_mask(int n, const double *ke, const double *mask, const double *in_, double *out_, int howmany) {
+  const cplx *in = (const cplx *)in_;
+  cplx *out = (cplx *)out_;
+  const size_t n3 = (size_t)n * n * n;
+  cplx *temp = malloc(sizeof(cplx) * n3 * howmany), *temp2 = malloc(sizeof(cplx) * n3 * howmany);
+  memset(out, 0, sizeof(cplx) * n3 * howmany);
+  for (int d = 0; d < 3; d++) {
+    for (int h = 0; h < howmany; h++)
+      for (size_t p = 0; p < n3; p++) {
+        const int x = (int)(p % n), y = (int)((p / n) % n), z = (int)(p / ((size_t)n * n));
+        cplx v = in[h * n3 + p];
+        v.re = v.re / mask[x]; v.im = v.im / mask[x];
+        v.re = v.re / mask[y]; v.im = v.im / mask[y];
+        v.re = v.re / mask[z]; v.im = v.im / mask[z];
+        temp[h * n3 + p] = v;
+      }
+    const size_t nnn = d == 0 ? 1 : (d == 1 ? (size_t)n : (size_t)n * n);
+    const size_t mmm = (d == 0 ? (size_t)n * n : (d == 1 ? (size_t)n : 1)) * howmany;
+#pragma omp parallel for schedule(static)
+    for (size_t mm = 0; mm < mmm; mm++)
+      for (int q = 0; q < n; q++)
+        for (size_t a = 0; a < nnn; a++) {
+          cplx acc = {0, 0};
+          for (int p = 0; p < n; p++) {
+            double kv = ke[(q - p) + n - 1];
+            cplx v = temp[a + nnn * (p + (size_t)n * mm)];
+            acc.re += v.re * kv; acc.im += v.im * kv;
+          }
+          temp2[a + nnn * (q + (size_t)n * mm)] = acc;
+        }
+    for (int h = 0; h < howmany; h++)
+      for (size_t p = 0; p < n3; p++) {
+        const int x = (int)(p % n), y = (int)((p / n) % n), z = (int)(p / ((size_t)n * n));
+        cplx v = temp2[h * n3 + p];
+        v = cscale(cscale(cscale(v, mask[x]), mask[y]), mask[z]);
+        out[h * n3 + p] = cadd(out[h * n3 + p], v);
+      }
+  }
   free(temp); free(temp2);
 }
 

@@ -270,3 +270,22 @@ def test_one_electron_matels(orc, torch_dev, cnorm, ecs):
         pm = s.pot_matel(1, N, phi, pot)
         assert relerr(pm, pm.T if cnorm else pm.conj().T) < 1e-12
 
+
+@pytest.mark.parametrize("k", [0, 3, 10])
+def test_mult_ke_mask(orc, torch_dev, k):
+    """maskflag kinetic apply (no ECS) vs the oracle, FP64 and FP32 matrix
+    cores, and through sparseops_matel."""
+    from sincdvr import SincDVR, inputs
+    n, dx = 21, 0.4
+    phi = inputs.gaussian_orbitals(n, dx, 3, 3, seed=8)
+    say("mult_ke mask k=%d" % k)
+    s = SincDVR(n, dx, 3, masknumpoints=k)
+    ke, _, _ = orc.sinedvr(n, dx)
+    m = orc.mask_function(n, k)
+    ref = orc.mult_ke_mask(n, ke, m, phi)
+    assert relerr(s.mult_ke(phi), ref) < 1e-12
+    op = s.sparseops_matel(1, 3, phi)
+    assert relerr(op, np.conj(phi).T @ ref) < 1e-12
+    s.set_precision(1)
+    assert relerr(s.mult_ke(phi), ref) < 2e-6
+

@@ -336,3 +336,34 @@ def test_one_matel_brute_force(orc, cnorm):
             bra = a[:, i] if cnorm else np.conj(a[:, i])
             assert abs(got[i, jj] - np.sum(bra * tb)) < 1e-12 * np.abs(tb).max() * np.abs(a).sum()
             assert abs(gotp[i, jj] - np.sum(bra * pot * b[:, col])) < 1e-12 * np.abs(b).sum() * 10
+
+
+def test_mask_function_closed_form(orc):
+    """maskfunction: 1 inside, (1 + cos(pi ii / (k + 1))) / 2 at the ii-th point
+    in from either end; overlapping ends multiply."""
+    n, k = 11, 3
+    m = orc.mask_function(n, k)
+    r = [(1 + np.cos(np.pi * ii / (k + 1))) / 2 for ii in range(1, k + 1)]
+    want = np.ones(n)
+    for ii in range(1, k + 1):
+        want[k - ii] *= r[ii - 1]
+        want[n - k + ii - 1] *= r[ii - 1]
+    assert np.array_equal(m, want)
+    assert m[0] < m[1] < m[2] < 1 and m[5] == 1
+    assert np.array_equal(orc.mask_function(n, 0), np.ones(n))
+    mo = orc.mask_function(4, 3)          # overlapping ends
+    assert mo[1] == ((1 + np.cos(np.pi * 2 / 4)) / 2) * ((1 + np.cos(np.pi * 1 / 4)) / 2)
+
+
+def test_mult_ke_mask_is_similarity(orc):
+    """Masked kinetic apply = M . T . M^-1 with M the diagonal product mask
+    (the per-axis mask factors of the other axes cancel)."""
+    n, d = 7, 0.4
+    ke, _, _ = orc.sinedvr(n, d)
+    m = orc.mask_function(n, 2)
+    M = np.einsum("i,j,k->kji", m, m, m).ravel()          # x fastest
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(n ** 3) + 1j * rng.standard_normal(n ** 3)
+    got = orc.mult_ke_mask(n, ke, m, x)[:, 0]
+    want = M * orc.mult_ke_dense(n, ke, x / M)
+    assert relerr(got, want) < 1e-13

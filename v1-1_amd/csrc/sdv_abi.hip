@@ -5,6 +5,7 @@
 // device-resident data.  No computation of the operators happens on the host:
 // the host builds index lists (pair bookkeeping) and launches kernels.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -196,6 +197,8 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
   SDV_REQUIRE(cfg->spacing > 0, SDV_ERR_ARG, "spacing must be positive");
   SDV_REQUIRE(cfg->numspf >= 1, SDV_ERR_ARG, "numspf must be >= 1");
   SDV_REQUIRE(cfg->precision == 0 || cfg->precision == 1, SDV_ERR_ARG, "precision must be 0 or 1");
+  SDV_REQUIRE(!cfg->maskflag || (cfg->masknumpoints >= 0 && cfg->masknumpoints <= cfg->numpoints), SDV_ERR_ARG,
+              "masknumpoints not allowed (SD/sincproject.f90:55-60)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     set_error("no HIP device visible");
@@ -231,6 +234,18 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
       if (hipMalloc(&c->invj, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
       if (rc == SDV_OK && hipMalloc(&c->sdiag, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
     }
+    if (rc == SDV_OK && cfg->maskflag && !cfg->scalingflag) {  // maskfunction, SD/init_sinc_new.f90:580-599
+      std::vector<double> m(n, 1.0);
+      const int k = cfg->masknumpoints;
+      for (int ii = 1; ii <= k; ++ii) {
+        const double rsum = (1.0 + std::cos(kPiD * ii / (k + 1))) / 2.0;
+        m[(k + 1 - ii) - 1] *= rsum;
+        m[(n - k + ii) - 1] *= rsum;
+      }
+      if (hipMalloc(&c->mask, n * sizeof(double)) != hipSuccess ||
+          hipMemcpy(c->mask, m.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = SDV_ERR_NOMEM;
+    }
     if (rc == SDV_OK) {
       hipLaunchKernelGGL(k_init_tables, dim3((2 * n + 255) / 256), dim3(256), 0, c->stream, n, cfg->spacing,
                          cfg->scalingflag ? 1 : 0, cfg->scalingtheta, cfg->scalingstretch, cfg->scalingdistance,
@@ -254,6 +269,7 @@ int sdv_free(sdv_ctx* c) {
   hipFree(c->ke);
   hipFree(c->invj);
   hipFree(c->sdiag);
+  hipFree(c->mask);
   for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2, &c->ob_ke,
                           &c->ob_scr, &c->h_pot})
     hipFree(b->p);
@@ -817,6 +833,7 @@ int sdv_mult_ke_dev(sdv_ctx* c, const void* in, void* out, int howmany, void* st
   a.ke = c->ke;
   a.invj = c->invj;
   a.sdiag = c->sdiag;
+  a.mask = c->mask;
   a.in = in;
   a.out = out;
   return ke_launch(a, c->precision, (hipStream_t)stream, &c->prof);
@@ -838,6 +855,7 @@ int sdv_sparseops_matel_dev(sdv_ctx* c, int low, int high, const void* inspfs1, 
   a.ke = c->ke;
   a.invj = c->invj;
   a.sdiag = c->sdiag;
+  a.mask = c->mask;
   a.in = (const char*)inspfs2 + col * (low - 1);
   a.out = c->ob_ke.p;
   SDV_TRY(ke_launch(a, c->precision, st, &c->prof));

@@ -160,6 +160,7 @@ struct KeArgsHost {
   const double* ke;          // 2n-1 stencil (device)
   const void* invj;          // 3 x n z64 per-axis 1/J (device) or null
   const void* sdiag;         // 3 x n z64 per-axis scalediag terms (device) or null
+  const double* mask = nullptr;  // n edge-mask values (device), or null: maskflag without ECS
   const void* in;            // z64 (n^3, howmany)
   void* out;
 };

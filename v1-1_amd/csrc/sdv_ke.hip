@@ -20,12 +20,26 @@ namespace sdv {
 
 constexpr int kKeThreads = 256;
 
+// maskflag (scalingflag 0): divide_mask / mult_mask of SD/coreproject.f90:1273-1319
+// divide by, then multiply with, the edge mask of the x, y and z coordinate of
+// grid point g, in that order.
+__device__ __forceinline__ z64 mask_divide(z64 v, const double* m, size_t g, int n) {
+  const double mx = m[g % n], my = m[(g / n) % n], mz = m[g / ((size_t)n * n)];
+  v = mk(v.re / mx, v.im / mx);
+  v = mk(v.re / my, v.im / my);
+  return mk(v.re / mz, v.im / mz);
+}
+__device__ __forceinline__ z64 mask_multiply(z64 v, const double* m, size_t g, int n) {
+  return scale(scale(scale(v, m[g % n]), m[(g / n) % n]), m[g / ((size_t)n * n)]);
+}
+
 // Axis-d pass.  Element (a, p, b) sits at a + s*p + s*n*b with s = n^d.
 // A tile is `la` consecutive inner indices a (d > 0) or `la` consecutive
 // lines b (d == 0), for one orbital (blockIdx.z).
 template <int AXIS>
 __global__ void __launch_bounds__(kKeThreads) k_ke_axis(int n, const double* __restrict__ ke,
                                                          const z64* __restrict__ invj, const z64* __restrict__ sdiag,
+                                                         const double* __restrict__ mask,
                                                          const z64* __restrict__ in, z64* __restrict__ out, int la) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* st = reinterpret_cast<double*>(smem_raw);      // 2n-1 stencil
@@ -65,6 +79,7 @@ __global__ void __launch_bounds__(kKeThreads) k_ke_axis(int n, const double* __r
     }
     z64 v = src[g];
     if (ij) v = v * ij[p];  // temp = in * invjacobian(:,d)   :1344-1346
+    if (mask) v = mask_divide(v, mask, g, n);
     buf[l * n + p] = v;
   }
   __syncthreads();
@@ -90,6 +105,7 @@ __global__ void __launch_bounds__(kKeThreads) k_ke_axis(int n, const double* __r
     }
     z64 r = mk(re, im);
     if (ij) r = r * ij[q];  // temp2 * invjacobian(:,d)   :1366-1368
+    if (mask) r = mask_multiply(r, mask, g, n);
     if (AXIS == 0) {
       z64 o = mk(0.0, 0.0);
       dst[g] = o + r;
@@ -145,6 +161,7 @@ constexpr int kKeMaxQT = 6;                  // q tiles of 16: n <= 96
 template <typename T, int AXIS>
 __global__ void __launch_bounds__(kKeThreads) k_ke_mma(int n, const double* __restrict__ ke,
                                                         const z64* __restrict__ invj, const z64* __restrict__ sdiag,
+                                                        const double* __restrict__ mask,
                                                         const z64* __restrict__ in, z64* __restrict__ out) {
   typedef typename Mma<T>::v4 v4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -186,6 +203,7 @@ __global__ void __launch_bounds__(kKeThreads) k_ke_mma(int n, const double* __re
       const size_t g = AXIS == 0 ? (size_t)p + (size_t)n * (b0 + l) : (a0 + l) + s * p + s * n * b0;
       z64 x = src[g];
       if (ij) x = x * ij[p];  // temp = in * invjacobian(:,d)   SD/coreproject.f90:1344-1346
+      if (mask) x = mask_divide(x, mask, g, n);  // divide_mask   :1355-1356
       v = mk((T)x.re, (T)x.im);
     }
     buf[l * pitch + p] = v;
@@ -243,6 +261,7 @@ __global__ void __launch_bounds__(kKeThreads) k_ke_mma(int n, const double* __re
     const cplx<T> rv = buf[ln * pitch + q];
     z64 r = mk((double)rv.re, (double)rv.im);
     if (ij) r = r * ij[q];  // temp2 * invjacobian(:,d)   :1366-1368
+    if (mask) r = mask_multiply(r, mask, g, n);  // mult_mask   :1373-1374
     if (AXIS == 0) {
       dst[g] = mk(0.0, 0.0) + r;
     } else if (AXIS == 1) {
@@ -270,13 +289,13 @@ static int ke_launch_mma(const KeArgsHost& a, hipStream_t st) {
   const z64* sd = (const z64*)a.sdiag;
   const unsigned nb0 = (unsigned)((n3 / n + kKeLines - 1) / kKeLines);
   hipLaunchKernelGGL((k_ke_mma<T, 0>), dim3(nb0, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
-                     (const z64*)a.in, (z64*)a.out);
+                     a.mask, (const z64*)a.in, (z64*)a.out);
   const unsigned nb1 = (unsigned)(((size_t)(n + kKeLines - 1) / kKeLines) * n);
   hipLaunchKernelGGL((k_ke_mma<T, 1>), dim3(nb1, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
-                     (const z64*)a.in, (z64*)a.out);
+                     a.mask, (const z64*)a.in, (z64*)a.out);
   const unsigned nb2 = (unsigned)(((size_t)n * n + kKeLines - 1) / kKeLines);
   hipLaunchKernelGGL((k_ke_mma<T, 2>), dim3(nb2, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
-                     (const z64*)a.in, (z64*)a.out);
+                     a.mask, (const z64*)a.in, (z64*)a.out);
   SDV_HIP_CHECK(hipGetLastError());
   return SDV_OK;
 }
@@ -303,19 +322,19 @@ int ke_launch(const KeArgsHost& a, int precision, hipStream_t st, Prof* prof) {
   {
     const unsigned nb = (unsigned)((n3 / n + la0 - 1) / la0);
     hipLaunchKernelGGL((k_ke_axis<0>), dim3(nb, 1, a.howmany), dim3(kKeThreads), sh0, st, n, a.ke, ij, sd,
-                       (const z64*)a.in, (z64*)a.out, la0);
+                       a.mask, (const z64*)a.in, (z64*)a.out, la0);
   }
   {
     const size_t s = n;
     const unsigned nb = (unsigned)(((s + la0 - 1) / la0) * (n3 / (s * n)));
     hipLaunchKernelGGL((k_ke_axis<1>), dim3(nb, 1, a.howmany), dim3(kKeThreads), sh0, st, n, a.ke, ij, sd,
-                       (const z64*)a.in, (z64*)a.out, la0);
+                       a.mask, (const z64*)a.in, (z64*)a.out, la0);
   }
   {
     const size_t s = (size_t)n * n;
     const unsigned nb = (unsigned)(((s + la0 - 1) / la0) * (n3 / (s * n)));
     hipLaunchKernelGGL((k_ke_axis<2>), dim3(nb, 1, a.howmany), dim3(kKeThreads), sh0, st, n, a.ke, ij, sd,
-                       (const z64*)a.in, (z64*)a.out, la0);
+                       a.mask, (const z64*)a.in, (z64*)a.out, la0);
   }
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_KE, e0, st);

@@ -222,6 +222,7 @@ static int launch_tick(sdv_server* v, int* launched) {
     a.ke = c->ke;
     a.invj = c->invj;
     a.sdiag = c->sdiag;
+    a.mask = c->mask;
     a.in = r.in;
     a.out = base + r.out_off;
     SDV_TRY(ke_launch(a, v->precision, v->stream, nullptr));

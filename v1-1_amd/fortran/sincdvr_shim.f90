@@ -25,7 +25,12 @@ module sincdvr_capi
      integer(c_int) :: toothnbig, toothnsmall, numspf, cnormflag, scalingflag
      real(c_double) :: scalingtheta, scalingstretch, scalingdistance, smoothness
      integer(c_int) :: notwoflag, debugflag, precision
+     integer(c_int) :: maskflag, masknumpoints
   end type sdv_config
+
+  ! maskflag / masknumpoints of sinc_params (SD/sinc_params.f90:73-74), set by
+  ! sincdvr_gpu_set_mask before sincdvr_gpu_init
+  integer :: sdv_maskflag = 0, sdv_masknumpoints = 0
 
   interface
      subroutine sdv_config_default(cfg) bind(c, name="sdv_config_default")
@@ -113,10 +118,21 @@ subroutine sincdvr_gpu_init(numpoints, spacing, numspf, cnormflag, scalingflag, 
   cfg%scalingstretch = scalingstretch; cfg%scalingdistance = scalingdistance; cfg%smoothness = smoothness
   cfg%toothnbig = toothnbig; cfg%toothnsmall = toothnsmall; cfg%notwoflag = notwoflag
   cfg%debugflag = debugflag; cfg%precision = precision
+  cfg%maskflag = sdv_maskflag; cfg%masknumpoints = sdv_masknumpoints
   call sdv_check(sdv_init(cfg, sdv_ctx_ptr), "sdv_init")
   sdv_totpoints = numpoints**3
   sdv_numspf = numspf
 end subroutine sincdvr_gpu_init
+
+! The kinetic apply's edge mask (maskflag, masknumpoints of sinc_params), for
+! the next sincdvr_gpu_init; used when scalingflag is 0.
+subroutine sincdvr_gpu_set_mask(maskflag, masknumpoints)
+  use sincdvr_capi
+  implicit none
+  integer, intent(in) :: maskflag, masknumpoints
+  sdv_maskflag = maskflag
+  sdv_masknumpoints = masknumpoints
+end subroutine sincdvr_gpu_set_mask
 
 
 module tinvsubmod

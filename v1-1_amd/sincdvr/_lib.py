@@ -51,6 +51,8 @@ class SdvConfig(ctypes.Structure):
         ("notwoflag", ctypes.c_int),
         ("debugflag", ctypes.c_int),
         ("precision", ctypes.c_int),
+        ("maskflag", ctypes.c_int),
+        ("masknumpoints", ctypes.c_int),
     ]
 
 

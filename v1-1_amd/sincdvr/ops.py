@@ -56,7 +56,7 @@ class SincDVR:
     stencil and ECS tables (the project state of myprojectmod / myparams)."""
 
     def __init__(self, numpoints, spacing, numspf, cnorm=False, scaling=None, toothnbig=240,
-                 toothnsmall=40, notwoflag=0, debugflag=0, precision=0):
+                 toothnsmall=40, notwoflag=0, debugflag=0, precision=0, masknumpoints=None):
         lib = _lib.load()
         cfg = _lib.SdvConfig()
         lib.sdv_config_default(ctypes.byref(cfg))
@@ -69,6 +69,9 @@ class SincDVR:
         cfg.notwoflag = notwoflag
         cfg.debugflag = debugflag
         cfg.precision = precision
+        if masknumpoints is not None:  # maskflag: kinetic apply through the edge mask (without ECS)
+            cfg.maskflag = 1
+            cfg.masknumpoints = int(masknumpoints)
         if scaling is not None:
             cfg.scalingflag = 1
             cfg.scalingtheta = scaling["theta"]
