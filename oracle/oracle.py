@@ -277,3 +277,26 @@ def one_matel(n, ke, in1, in2, low=1, high=None, pot=None, cnorm=False, invjac=N
         w = cols * np.asarray(pot, dtype=np.float64)[:, None]
     a = in1 if cnorm else np.conj(in1)
     return np.asfortranarray(a.T @ w)
+
+
+def actreduced00(n, ke, inspfs, denmat, reducedpot, pot=None, invdenmat=None, timefac=1.0, projspfs=None,
+                 cnorm=False, invjac=None, scalediag=None, mask=None):
+    """S/derivs.f90:629-852 for the sinc-DVR project (one bond point): the
+    kinetic apply is the C restatement (with ECS or the edge mask), the rest
+    numpy, in the reference's order."""
+    phi = np.asfortranarray(inspfs, dtype=np.complex128)
+    w = phi @ np.asarray(denmat)                                   # spfinvrsq = spfinvr
+    if mask is not None:
+        s = mult_ke_mask(n, ke, mask, w)
+    else:
+        s = mult_ke(n, ke, w, invjac, scalediag)
+    if pot is not None:
+        s = s + w * np.asarray(pot)[:, None]
+    s = s + mult_reducedpot(1, phi.shape[1], phi, reducedpot)
+    if invdenmat is not None:
+        s = timefac * (s @ np.asarray(invdenmat))
+    if projspfs is not None:
+        pj = np.asarray(projspfs)
+        d = (pj if cnorm else np.conj(pj)).T @ s                    # mydot(j, i) = dot(pro_j, s_i)
+        s = s - pj @ d
+    return np.asfortranarray(s)

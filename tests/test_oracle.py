@@ -367,3 +367,26 @@ def test_mult_ke_mask_is_similarity(orc):
     got = orc.mult_ke_mask(n, ke, m, x)[:, 0]
     want = M * orc.mult_ke_dense(n, ke, x / M)
     assert relerr(got, want) < 1e-13
+
+
+def test_actreduced00_restatement(orc):
+    """Orbital operator restatement: with denmat = 1, no potentials and no
+    mean field it is the kinetic apply; the projected result is orthogonal to
+    the projection orbitals; dentime multiplies by timefac . invdenmat."""
+    n, d, N = 5, 0.4, 3
+    ke, _, _ = orc.sinedvr(n, d)
+    rng = np.random.default_rng(8)
+    phi = np.asfortranarray(rng.standard_normal((n ** 3, N)) + 1j * rng.standard_normal((n ** 3, N)))
+    q, _ = np.linalg.qr(phi)
+    zero_rp = np.zeros((n ** 3, N, N), dtype=complex, order="F")
+    out = orc.actreduced00(n, ke, phi, np.eye(N), zero_rp)
+    assert relerr(out, orc.mult_ke(n, ke, phi)) < 1e-14
+    dm = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    rp = np.asfortranarray(rng.standard_normal((n ** 3, N, N)) + 1j * rng.standard_normal((n ** 3, N, N)))
+    pot = rng.standard_normal(n ** 3)
+    full = orc.actreduced00(n, ke, phi, dm, rp, pot=pot, projspfs=q)
+    assert np.abs(np.conj(q).T @ full).max() < 1e-12 * np.abs(full).max()
+    inv = rng.standard_normal((N, N)) + 0j
+    a = orc.actreduced00(n, ke, phi, dm, rp, pot=pot)
+    b = orc.actreduced00(n, ke, phi, dm, rp, pot=pot, invdenmat=inv, timefac=-1j)
+    assert relerr(b, -1j * (a @ inv)) < 1e-14

@@ -271,7 +271,8 @@ int sdv_free(sdv_ctx* c) {
   hipFree(c->sdiag);
   hipFree(c->mask);
   for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2, &c->ob_ke,
-                          &c->ob_scr, &c->h_pot})
+                          &c->ob_scr, &c->h_pot, &c->ar_w, &c->ar_rp, &c->ar_s, &c->ar_s2, &c->ar_dots,
+                          &c->h_small, &c->h_out3})
     hipFree(b->p);
   for (auto& kv : c->plans) hipFree(kv.second.dev);
   for (auto& r : c->prof.recs) {
@@ -875,6 +876,50 @@ int sdv_pot_matel_dev(sdv_ctx* c, int low, int high, const void* inspfs1, const 
                     c->cfg.cnormflag ? 0 : 1, c->ob_scr.p, potmatel, c->N, (hipStream_t)stream);
 }
 
+int sdv_actreduced00_dev(sdv_ctx* c, const void* inspfs, const void* projspfs, const void* denmat,
+                         const void* invdenmat, const void* reducedpot, const double* pot, int dentimeflag,
+                         int projflag, double timefac_re, double timefac_im, void* outspfs, void* stream) {
+  SDV_REQUIRE(c && inspfs && denmat && reducedpot && outspfs, SDV_ERR_ARG, "null argument");
+  SDV_REQUIRE(!dentimeflag || invdenmat, SDV_ERR_ARG, "dentimeflag needs invdenmat");
+  SDV_REQUIRE(!projflag || projspfs, SDV_ERR_ARG, "projflag needs projspfs");
+  hipStream_t st = (hipStream_t)stream;
+  const int N = c->N, n3 = c->n3;
+  const size_t cols = (size_t)n3 * N * sizeof(z64);
+  SDV_TRY(ensure(c->ar_w, cols));
+  SDV_TRY(ensure(c->ob_ke, cols));
+  SDV_TRY(ensure(c->ar_rp, cols));
+  SDV_TRY(ensure(c->ar_s, cols));
+  SDV_TRY(ensure(c->ar_s2, cols));
+  SDV_TRY(ensure(c->ar_dots, (size_t)N * N * sizeof(z64)));
+  SDV_TRY(ensure(c->ob_scr, dot_scratch_bytes(n3, N, N)));
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  // spfinvrsq = spfinvr = inspfs . denmat   (S/derivs.f90:670-688, bondpoints(1) = 1)
+  SDV_TRY(orb_gemm_launch(n3, N, N, inspfs, denmat, N, one, zero, c->ar_w.p, st));
+  KeArgsHost a;
+  a.n = c->n;
+  a.howmany = N;
+  a.ke = c->ke;
+  a.invj = c->invj;
+  a.sdiag = c->sdiag;
+  a.mask = c->mask;
+  a.in = c->ar_w.p;
+  a.out = c->ob_ke.p;
+  SDV_TRY(ke_launch(a, c->precision, st, &c->prof));                                       // :691-693
+  SDV_TRY(mult_reducedpot_launch(n3, N, N, inspfs, reducedpot, c->ar_rp.p, st));           // :762-764
+  SDV_TRY(orb_sum_launch(n3, N, c->ob_ke.p, pot, c->ar_w.p, c->ar_rp.p, c->ar_s.p, st));  // + mult_pot :698
+  void* cur = c->ar_s.p;
+  if (dentimeflag) {  // spfmult . invdenmat * timefac   :780-787
+    SDV_TRY(orb_gemm_launch(n3, N, N, cur, invdenmat, N, mk(timefac_re, timefac_im), zero, c->ar_s2.p, st));
+    cur = c->ar_s2.p;
+  }
+  SDV_HIP_CHECK(hipMemcpyAsync(outspfs, cur, cols, hipMemcpyDeviceToDevice, st));
+  if (projflag) {  // out = spfmult - projspfs . <projspfs | spfmult>   project00, S/proputils.f90:10-60
+    SDV_TRY(dot_launch(n3, N, N, projspfs, cur, nullptr, c->cfg.cnormflag ? 0 : 1, c->ob_scr.p, c->ar_dots.p, N, st));
+    SDV_TRY(orb_gemm_launch(n3, N, N, projspfs, c->ar_dots.p, N, mk(-1.0, 0.0), one, outspfs, st));
+  }
+  return SDV_OK;
+}
+
 int sdv_mult_reducedpot_dev(sdv_ctx* c, int first, int last, const void* inspfs, void* outspfs,
                             const void* reducedpot, void* stream) {
   SDV_REQUIRE(c && inspfs && outspfs && reducedpot, SDV_ERR_ARG, "null argument");
@@ -989,6 +1034,29 @@ int sdv_pot_matel(sdv_ctx* c, int low, int high, const double* inspfs1, const do
   SDV_TRY(ensure(c->h_out2, ob));
   SDV_TRY(sdv_pot_matel_dev(c, low, high, d1, d2, (const double*)dp, c->h_out2.p, c->stream));
   SDV_HIP_CHECK(hipMemcpyAsync(potmatel, c->h_out2.p, ob, hipMemcpyDefault, c->stream));
+  SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return SDV_OK;
+}
+
+int sdv_actreduced00(sdv_ctx* c, const double* inspfs, const double* projspfs, const double* denmat,
+                     const double* invdenmat, const double* reducedpot, const double* pot, int dentimeflag,
+                     int projflag, double timefac_re, double timefac_im, double* outspfs) {
+  SDV_REQUIRE(c && inspfs && denmat && reducedpot && outspfs, SDV_ERR_ARG, "null argument");
+  const size_t pb = (size_t)c->n3 * c->N * sizeof(z64), nn = (size_t)c->N * c->N * sizeof(z64);
+  const void *din = nullptr, *dproj = nullptr, *drp = nullptr, *dpot = nullptr;
+  SDV_TRY(stage_in(c, c->h_in1, inspfs, pb, &din));
+  if (projflag && projspfs) SDV_TRY(stage_in(c, c->h_in2, projspfs, pb, &dproj));
+  SDV_TRY(stage_in(c, c->h_out2, reducedpot, pb * c->N, &drp));
+  if (pot) SDV_TRY(stage_in(c, c->h_pot, pot, (size_t)c->n3 * sizeof(double), &dpot));
+  SDV_TRY(ensure(c->h_small, 2 * nn));
+  char* dm = (char*)c->h_small.p;
+  SDV_HIP_CHECK(hipMemcpyAsync(dm, denmat, nn, hipMemcpyHostToDevice, c->stream));
+  if (dentimeflag && invdenmat) SDV_HIP_CHECK(hipMemcpyAsync(dm + nn, invdenmat, nn, hipMemcpyHostToDevice, c->stream));
+  SDV_TRY(ensure(c->h_out3, pb));
+  SDV_TRY(sdv_actreduced00_dev(c, din, dproj, dm, (dentimeflag && invdenmat) ? dm + nn : nullptr, drp,
+                               (const double*)dpot, dentimeflag, projflag, timefac_re, timefac_im, c->h_out3.p,
+                               c->stream));
+  SDV_HIP_CHECK(hipMemcpyAsync(outspfs, c->h_out3.p, pb, hipMemcpyDeviceToHost, c->stream));
   SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
   return SDV_OK;
 }

@@ -172,6 +172,13 @@ size_t dot_scratch_bytes(int n3, int na, int nb);
 int dot_launch(int n3, int na, int nb, const void* A, const void* B, const double* w, int conj_a, void* scratch,
                void* out, int ld, hipStream_t st);
 
+// Orbital operator pieces: out = alpha A C + beta out (A: n3 x nin device
+// columns, C: nin x nout device, column-major ldc); s = ke + pot w + rp.
+int orb_gemm_launch(int n3, int nin, int nout, const void* A, const void* C, int ldc, z64 alpha, z64 beta, void* out,
+                    hipStream_t st);
+int orb_sum_launch(int n3, int ncols, const void* ke, const double* pot, const void* w, const void* rp, void* s,
+                   hipStream_t st);
+
 // Mean field
 int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, hipStream_t st);
 int mult_reducedpot_launch(int n3, int N, int nout, const void* inspfs, const void* rp, void* out,

@@ -98,3 +98,82 @@ int dot_launch(int n3, int na, int nb, const void* A, const void* B, const doubl
 }
 
 }  // namespace sdv
+
+namespace sdv {
+
+// ---------------------------------------------------------- orbital operator
+// out(x, j) = alpha * sum_i A(x, i) C(i + ldc j) + beta * out(x, j) for j in
+// [0, nout), i in [0, nin); C is a small complex matrix (column-major, ldc).
+// One thread per grid point and 16 output columns; C's tile in LDS.
+constexpr int kOgCols = 16;
+
+__global__ void __launch_bounds__(256) k_orb_gemm(int n3, int nin, int nout, const z64* __restrict__ A,
+                                                   const z64* __restrict__ C, int ldc, z64 alpha, z64 beta,
+                                                   z64* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  z64* cs = reinterpret_cast<z64*>(smem_raw);  // [nin][kOgCols]
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  const int j0 = blockIdx.y * kOgCols;
+  const int nj = min(kOgCols, nout - j0);
+  for (int e = threadIdx.x; e < nin * kOgCols; e += 256) {
+    const int i = e / kOgCols, j = e - i * kOgCols;
+    cs[e] = j < nj ? C[(size_t)i + (size_t)ldc * (j0 + j)] : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  if (x >= n3) return;
+  z64 acc[kOgCols];
+#pragma unroll
+  for (int j = 0; j < kOgCols; ++j) acc[j] = mk(0.0, 0.0);
+  for (int i = 0; i < nin; ++i) {
+    const z64 a = A[(size_t)x + (size_t)n3 * i];
+#pragma unroll
+    for (int j = 0; j < kOgCols; ++j) {
+      const z64 c = cs[i * kOgCols + j];
+      acc[j].re = fma(a.re, c.re, fma(-a.im, c.im, acc[j].re));
+      acc[j].im = fma(a.re, c.im, fma(a.im, c.re, acc[j].im));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kOgCols; ++j) {
+    if (j < nj) {
+      z64* o = out + (size_t)x + (size_t)n3 * (j0 + j);
+      z64 r = alpha * acc[j];
+      if (beta.re != 0.0 || beta.im != 0.0) r = r + beta * (*o);
+      *o = r;
+    }
+  }
+}
+
+int orb_gemm_launch(int n3, int nin, int nout, const void* A, const void* C, int ldc, z64 alpha, z64 beta, void* out,
+                    hipStream_t st) {
+  if (nout <= 0) return SDV_OK;
+  SDV_REQUIRE(nin >= 1 && nin <= 512, SDV_ERR_UNSUPPORTED, "orbital count outside 1..512");
+  const size_t sh = (size_t)nin * kOgCols * sizeof(z64);
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_orb_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  hipLaunchKernelGGL(k_orb_gemm, dim3((n3 + 255) / 256, (nout + kOgCols - 1) / kOgCols), dim3(256), sh, st, n3, nin,
+                     nout, (const z64*)A, (const z64*)C, ldc, alpha, beta, (z64*)out);
+  SDV_HIP_CHECK(hipGetLastError());
+  return SDV_OK;
+}
+
+// s(x, j) = ke(x, j) + pot(x) w(x, j) + rp(x, j)   (actreduced00's spfmult sum)
+__global__ void k_orb_sum(size_t total, int n3, const z64* __restrict__ ke, const double* __restrict__ pot,
+                          const z64* __restrict__ w, const z64* __restrict__ rp, z64* __restrict__ s) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  z64 v = ke[e];
+  if (pot) v = v + scale(w[e], pot[e % n3]);
+  s[e] = v + rp[e];
+}
+
+int orb_sum_launch(int n3, int ncols, const void* ke, const double* pot, const void* w, const void* rp, void* s,
+                   hipStream_t st) {
+  const size_t total = (size_t)n3 * ncols;
+  if (!total) return SDV_OK;
+  hipLaunchKernelGGL(k_orb_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, n3, (const z64*)ke,
+                     pot, (const z64*)w, (const z64*)rp, (z64*)s);
+  SDV_HIP_CHECK(hipGetLastError());
+  return SDV_OK;
+}
+
+}  // namespace sdv

@@ -23,6 +23,7 @@ EXPORTS = [
     "sdv_twoereduced", "sdv_twoereduced_dev", "sdv_profile", "sdv_profile_read", "sdv_nuclear_potential",
     "sdv_get_kernel_octant", "sdv_get_ke_stencil", "sdv_circulant_edge",
     "sdv_sparseops_matel", "sdv_sparseops_matel_dev", "sdv_pot_matel", "sdv_pot_matel_dev",
+    "sdv_actreduced00", "sdv_actreduced00_dev",
     "sdv_circulant_sizes", "sdv_set_circulant", "sdv_set_pair_batch",
     "sdv_batch_create", "sdv_batch_set_schedule", "sdv_batch_run", "sdv_batch_capture", "sdv_batch_stats",
     "sdv_batch_free", "sdv_batch_profile", "sdv_batch_profile_read",
@@ -112,6 +113,8 @@ def load():
     lib.sdv_sparseops_matel_dev.argtypes = [vp, i, i, vp, vp, vp, vp]
     lib.sdv_pot_matel.argtypes = [vp, i, i, vp, vp, vp, vp]
     lib.sdv_pot_matel_dev.argtypes = [vp, i, i, vp, vp, vp, vp, vp]
+    lib.sdv_actreduced00.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, d, d, vp]
+    lib.sdv_actreduced00_dev.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, d, d, vp, vp]
     lib.sdv_circulant_sizes.argtypes = [vp, i]
     lib.sdv_set_circulant.argtypes = [vp, i]
     lib.sdv_set_pair_batch.argtypes = [vp, i]

@@ -263,6 +263,39 @@ class SincDVR:
                                                ctypes.c_void_p(pt.ctypes.data), _hptr(out)))
         return out
 
+    # ------------------------------------------------ orbital operator apply
+    def actreduced00(self, inspfs, denmat, reducedpot, pot=None, invdenmat=None, timefac=None, projspfs=None):
+        """S/derivs.f90:629-852 over all orbitals (see sdv_actreduced00):
+        out = (1 - P)[timefac (T w + pot w + mult_reducedpot) invdenmat],
+        w = inspfs . denmat; P projects on projspfs (flavour dot).  The
+        dentime step runs when invdenmat is given, the projection when
+        projspfs is given.  numpy (Fortran layout) or torch (reversed shape)."""
+        tf = complex(1.0 if timefac is None else timefac)
+        dent = invdenmat is not None
+        proj = projspfs is not None
+        if _is_torch(inspfs):
+            p = inspfs.contiguous()
+            dm = denmat.contiguous()
+            out = torch.empty_like(p)
+            args = [_dptr(p), _dptr(projspfs.contiguous()) if proj else None, _dptr(dm),
+                    _dptr(invdenmat.contiguous()) if dent else None, _dptr(reducedpot.contiguous()),
+                    ctypes.c_void_p(pot.contiguous().data_ptr()) if pot is not None else None]
+            _lib.check(self._lib.sdv_actreduced00_dev(self._h, *args, int(dent), int(proj), tf.real, tf.imag,
+                                                      _dptr(out), _stream()))
+            return out
+        p = np.asfortranarray(inspfs, dtype=np.complex128)
+        dm = np.asfortranarray(denmat, dtype=np.complex128)
+        rp = np.asfortranarray(reducedpot, dtype=np.complex128)
+        pj = np.asfortranarray(projspfs, dtype=np.complex128) if proj else None
+        idm = np.asfortranarray(invdenmat, dtype=np.complex128) if dent else None
+        pt = np.ascontiguousarray(pot, dtype=np.float64) if pot is not None else None
+        out = np.zeros_like(p, order="F")
+        _lib.check(self._lib.sdv_actreduced00(self._h, _hptr(p), _hptr(pj) if proj else None, _hptr(dm),
+                                              _hptr(idm) if dent else None, _hptr(rp),
+                                              ctypes.c_void_p(pt.ctypes.data) if pt is not None else None,
+                                              int(dent), int(proj), tf.real, tf.imag, _hptr(out)))
+        return out
+
     # -------------------------------------------------------- mult_reducedpot
     def mult_reducedpot(self, firstspf, lastspf, inspfs, reducedpot):
         """SD/coreproject.f90:1200-1235."""
