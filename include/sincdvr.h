@@ -141,20 +141,26 @@ int sdv_pot_matel_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs1
 
 /* The orbital operator of the propagator's Krylov matvec, actreduced00
  * (S/derivs.f90:629-852) for the sinc-DVR project (one bond point, no frozen
- * orbitals, no pulse, scalar or constraint terms), over all numspf orbitals:
- *   w       = inspfs . denmat                        (spfinvr = spfinvrsq)
- *   spfmult = mult_ke(w) + pot * w + mult_reducedpot(inspfs, reducedpot)
+ * orbitals, scalar or constraint terms; the pulse in the length gauge), over
+ * all numspf orbitals:
+ *   w       = inspfs . denmat                        (spfinvr = spfinvrsq = spfr)
+ *   spfmult = mult_ke(w) + pot * w + pulse(w) + mult_reducedpot(inspfs, reducedpot)
+ *             pulse = lenmultiply (S/proputils.f90:157-182): the field values
+ *             tdpot = {x.re, x.im, y.re, y.im, z.re, z.im} (host, or null for
+ *             no field) times the dipoles (grid coordinates, complex-scaled
+ *             under ECS, SD/init_sinc_new.f90:527,619-645)
  *   dentimeflag: spfmult = timefac * spfmult . invdenmat
  *   projflag:    out = spfmult - projspfs . <projspfs | spfmult>   (project00,
  *                S/proputils.f90:10-60, flavour dot), else out = spfmult.
  * denmat, invdenmat: numspf x numspf complex, column-major; reducedpot:
  * (totpoints, numspf, numspf); pot: real (totpoints) or null. */
 int sdv_actreduced00(sdv_ctx* ctx, const double* inspfs, const double* projspfs, const double* denmat,
-                     const double* invdenmat, const double* reducedpot, const double* pot, int dentimeflag,
-                     int projflag, double timefac_re, double timefac_im, double* outspfs);
+                     const double* invdenmat, const double* reducedpot, const double* pot, const double* tdpot,
+                     int dentimeflag, int projflag, double timefac_re, double timefac_im, double* outspfs);
 int sdv_actreduced00_dev(sdv_ctx* ctx, const void* inspfs, const void* projspfs, const void* denmat,
-                         const void* invdenmat, const void* reducedpot, const double* pot, int dentimeflag,
-                         int projflag, double timefac_re, double timefac_im, void* outspfs, void* stream);
+                         const void* invdenmat, const void* reducedpot, const double* pot, const double* tdpot,
+                         int dentimeflag, int projflag, double timefac_re, double timefac_im, void* outspfs,
+                         void* stream);
 
 /* Tuning (no counterpart in the reference, whose circulant edge is fixed at
  * 2*numpoints, SD/coreproject.f90:389-391).  Any circulant edge M = 2m with

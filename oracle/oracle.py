@@ -279,19 +279,48 @@ def one_matel(n, ke, in1, in2, low=1, high=None, pot=None, cnorm=False, invjac=N
     return np.asfortranarray(a.T @ w)
 
 
+def dipoles_1d(n, spacing, ecs=None):
+    """Per-axis dipole values of the cubic grid: the sine-DVR points
+    (get_one_dipole, SINCDVR/init_sinc_new.f90:630-645), replaced by the ECS
+    scale function F(x) when ecs = (theta, stretch, dist, smooth) is given
+    (dipoles = scalefunction, :527)."""
+    _, _, pts = sinedvr(n, spacing)
+    if ecs is None:
+        return pts.astype(np.complex128)
+    return np.array([ecs_point(x, *ecs)[0] for x in pts], dtype=np.complex128)
+
+
+def lenmultiply(n, w, dip, tdpot):
+    """S/proputils.f90:157-182 with SINCDVR/coreproject.f90:820-884 (realflag
+    0): ((0 + (w dz) Ez) + (w dy) Ey) + (w dx) Ex, a term skipped when its
+    field value is zero.  dip: per-axis values (x fastest in the point index)."""
+    n3 = n ** 3
+    g = np.arange(n3)
+    d = [np.asarray(dip)[g % n], np.asarray(dip)[(g // n) % n], np.asarray(dip)[g // (n * n)]]
+    out = np.zeros_like(w)
+    for axis in (2, 1, 0):
+        t = complex(tdpot[axis])
+        if t != 0:
+            out = out + (w * d[axis][:, None]) * t
+    return out
+
+
 def actreduced00(n, ke, inspfs, denmat, reducedpot, pot=None, invdenmat=None, timefac=1.0, projspfs=None,
-                 cnorm=False, invjac=None, scalediag=None, mask=None):
+                 cnorm=False, invjac=None, scalediag=None, mask=None, tdpot=None, dip=None):
     """S/derivs.f90:629-852 for the sinc-DVR project (one bond point): the
     kinetic apply is the C restatement (with ECS or the edge mask), the rest
-    numpy, in the reference's order."""
+    numpy, in the reference's order.  tdpot = (Ex, Ey, Ez) with dip
+    (dipoles_1d) adds the length-gauge pulse (:740-746)."""
     phi = np.asfortranarray(inspfs, dtype=np.complex128)
-    w = phi @ np.asarray(denmat)                                   # spfinvrsq = spfinvr
+    w = phi @ np.asarray(denmat)                                   # spfinvrsq = spfinvr = spfr
     if mask is not None:
         s = mult_ke_mask(n, ke, mask, w)
     else:
         s = mult_ke(n, ke, w, invjac, scalediag)
     if pot is not None:
         s = s + w * np.asarray(pot)[:, None]
+    if tdpot is not None:
+        s = s + lenmultiply(n, w, dip, tdpot)
     s = s + mult_reducedpot(1, phi.shape[1], phi, reducedpot)
     if invdenmat is not None:
         s = timefac * (s @ np.asarray(invdenmat))

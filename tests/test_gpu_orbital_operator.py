@@ -63,6 +63,34 @@ def test_actreduced00_vs_oracle(orc, torch_dev, sysm, dent, proj):
     assert np.array_equal(dev.cpu().numpy().T, got)
 
 
+@pytest.mark.parametrize("ecs", [False, True])
+def test_actreduced00_pulse_vs_oracle(orc, torch_dev, sysm, ecs):
+    """Length-gauge pulse term of the orbital operator (field on all three
+    axes, and on z only) against the oracle, with and without ECS dipoles."""
+    torch = torch_dev
+    from sincdvr import SincDVR
+    m = sysm
+    n, dx, N = m["n"], 0.4, m["N"]
+    sc = dict(theta=0.35, stretch=1.1, dist=1.5, smooth=0.8) if ecs else None
+    s = SincDVR(n, dx, N, scaling=sc) if ecs else m["s"]
+    ij = sd = None
+    if ecs:
+        ij, sd = orc.ecs_arrays(n, dx, sc["theta"], sc["stretch"], sc["dist"], sc["smooth"])
+    dip = orc.dipoles_1d(n, dx, (sc["theta"], sc["stretch"], sc["dist"], sc["smooth"]) if ecs else None)
+    for td in [(0.02 - 0.01j, -0.03, 0.05 + 0.02j), (0.0, 0.0, 0.04)]:
+        say("actreduced00 pulse ecs=%s field=%s" % (ecs, td))
+        got = s.actreduced00(m["phi"], m["denmat"], m["rp"], pot=m["pot"], invdenmat=m["invden"], timefac=-1j,
+                             projspfs=m["phi"], tdpot=td)
+        ref = orc.actreduced00(n, m["ke"], m["phi"], m["denmat"], m["rp"], pot=m["pot"], invdenmat=m["invden"],
+                               timefac=-1j, projspfs=m["phi"], invjac=ij, scalediag=sd, tdpot=td, dip=dip)
+        assert relerr(got, ref) < 1e-12
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).cuda()      # noqa: E731
+        dev = s.actreduced00(T(m["phi"]), T(m["denmat"]), T(m["rp"]), pot=torch.from_numpy(m["pot"]).cuda(),
+                             invdenmat=T(m["invden"]), timefac=-1j, projspfs=T(m["phi"]), tdpot=td)
+        torch.cuda.synchronize()
+        assert np.array_equal(dev.cpu().numpy().T, got)
+
+
 def test_krylov_step_matches_oracle(orc, torch_dev, sysm):
     """One Arnoldi / exponential step of the orbital equation with the device
     operator (vectors resident, FP64 coefficients on the host) against the same

@@ -390,3 +390,50 @@ def test_actreduced00_restatement(orc):
     a = orc.actreduced00(n, ke, phi, dm, rp, pot=pot)
     b = orc.actreduced00(n, ke, phi, dm, rp, pot=pot, invdenmat=inv, timefac=-1j)
     assert relerr(b, -1j * (a @ inv)) < 1e-14
+
+
+def test_dipoles_and_lenmultiply(orc):
+    """Length-gauge pulse restatement.  Dipoles: the centred sine-DVR grid
+    (spacing apart, summing to zero); under ECS, F(x) = x inside the unscaled
+    region, the closed-form straight line beyond the ramp, and dF/dx = J of the
+    scaling (central differences).  lenmultiply: the x, y and z terms of a
+    separable orbital f(x) g(y) h(z) are x f g h, f y g h, f g z h (an axis
+    swap fails), the sum is linear in the field, and <phi|x|phi> of a
+    Gaussian centred at x0 well inside the grid is x0."""
+    n, d = 9, 0.5
+    dip = orc.dipoles_1d(n, d)
+    assert np.allclose(np.diff(dip.real), d, atol=1e-14) and abs(dip.sum()) < 1e-13
+    theta, stretch, R0, S = 0.4, 1.2, 1.0, 0.75
+    dipe = orc.dipoles_1d(n, d, (theta, stretch, R0, S))
+    fac = stretch * np.exp(1j * theta)
+    for x, f in zip(dip.real, dipe):
+        if abs(x) <= R0:
+            assert f == x
+        elif abs(x) >= R0 + S:
+            assert abs(f - (x + (fac - 1) * (x - np.sign(x) * (R0 + S / 2)))) < 1e-14
+    for x in (-1.6, -1.3, 1.2, 1.5, 1.7):
+        h = 1e-5
+        dF = (orc.ecs_point(x + h, theta, stretch, R0, S)[0] - orc.ecs_point(x - h, theta, stretch, R0, S)[0]) / (2 * h)
+        assert abs(dF - orc.ecs_point(x, theta, stretch, R0, S)[1]) < 1e-8
+    rng = np.random.default_rng(5)
+    f, g, h = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3)]
+    phi = np.einsum("k,j,i->kji", h, g, f).reshape(-1, 1)           # x fastest in the point index
+    for axis, (a, b, c) in enumerate([(dip * f, g, h), (f, dip * g, h), (f, g, dip * h)]):
+        t = [0, 0, 0]
+        t[axis] = 1.0
+        want = np.einsum("k,j,i->kji", c, b, a).reshape(-1, 1)
+        assert relerr(orc.lenmultiply(n, phi, dip, t), want) < 1e-14
+    E1, E2 = (0.3, -0.2j, 0.1 + 0.4j), (-0.5j, 0.7, 0.2)
+    lhs = orc.lenmultiply(n, phi, dipe, [a + b for a, b in zip(E1, E2)])
+    assert relerr(lhs, orc.lenmultiply(n, phi, dipe, E1) + orc.lenmultiply(n, phi, dipe, E2)) < 1e-14
+    n2, x0 = 25, 0.6
+    dg = orc.dipoles_1d(n2, 0.3).real
+    X, Y, Z = np.meshgrid(dg, dg, dg, indexing="ij")
+    gau = np.exp(-2 * ((X - x0) ** 2 + Y ** 2 + Z ** 2)).transpose(2, 1, 0).reshape(-1, 1) + 0j
+    xphi = orc.lenmultiply(n2, gau, dg + 0j, [1.0, 0, 0])
+    assert abs(np.vdot(gau, xphi) / np.vdot(gau, gau) - x0) < 1e-10
+    ke, _, _ = orc.sinedvr(n, d)
+    zero_rp = np.zeros((n ** 3, 1, 1), dtype=complex, order="F")
+    a = orc.actreduced00(n, ke, phi, np.eye(1), zero_rp)
+    b = orc.actreduced00(n, ke, phi, np.eye(1), zero_rp, tdpot=E1, dip=dipe)
+    assert relerr(b - a, orc.lenmultiply(n, phi, dipe, E1)) < 1e-13

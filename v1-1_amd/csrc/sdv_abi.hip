@@ -42,8 +42,11 @@ __device__ double ddjscaled_d(double x) {
   return 9.0 / 4.0 * kPiD * kPiD * s * s * c;
 }
 
+// dip(i) = dipole (coordinate) of grid index i on one axis: the grid point,
+// or the complex-scaled coordinate F(x) with ECS (get_dipoles,
+// SD/init_sinc_new.f90:527,619-645; F of :5-30).
 __global__ void k_init_tables(int n, double spacing, int ecs, double theta, double stretch, double R0, double S,
-                              double* ke, z64* invj, z64* sdiag) {
+                              double* ke, z64* invj, z64* sdiag, z64* dip) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < 2 * n - 1) {
     const int i = t - (n - 1);
@@ -54,6 +57,10 @@ __global__ void k_init_tables(int n, double spacing, int ecs, double theta, doub
       k = 2.0 / ((double)i * (double)i) * ((i & 1) ? -1.0 : 1.0);
     ke[t] = k / 2.0 / (spacing * spacing);
   }
+  if (!ecs && t < n) {
+    const double p1 = 1 * spacing, pn = n * spacing;
+    dip[t] = mk((t + 1) * spacing - (p1 + pn) / 2.0, 0.0);
+  }
   if (ecs && t < n) {
     const double p1 = 1 * spacing, pn = n * spacing;
     const double x = (t + 1) * spacing - (p1 + pn) / 2.0;
@@ -61,25 +68,36 @@ __global__ void k_init_tables(int n, double spacing, int ecs, double theta, doub
     const z64 fm1 = mk(fac.re - 1.0, fac.im);
     z64 j, dj, ddj;
     const z64 zero = mk(0.0, 0.0);
-    if (x <= -(R0 + S) || x >= R0 + S) {
+    z64 f;
+    if (x <= -(R0 + S)) {
+      f = mk(x, 0.0) + fm1 * mk(x + R0 + S / 2.0, 0.0);
+      j = fac;
+      dj = zero;
+      ddj = zero;
+    } else if (x >= R0 + S) {
+      f = mk(x, 0.0) + fm1 * mk(x - R0 - S / 2.0, 0.0);
       j = fac;
       dj = zero;
       ddj = zero;
     } else if (fabs(x) <= R0) {
+      f = mk(x, 0.0);
       j = mk(1.0, 0.0);
       dj = zero;
       ddj = zero;
     } else if (x >= R0) {
       const double xx = (x - R0) / S;
+      f = mk(x, 0.0) + scale(fm1, S * fscaled_d(xx));
       j = mk(1.0, 0.0) + scale(fm1, jscaled_d(xx));
       dj = scale(fm1, djscaled_d(xx) / S);
       ddj = scale(fm1, ddjscaled_d(xx) / (S * S));
     } else {
       const double xx = (-x - R0) / S;
+      f = mk(x, 0.0) - scale(fm1, S * fscaled_d(xx));
       j = mk(1.0, 0.0) + scale(fm1, jscaled_d(xx));
       dj = scale(mk(-fm1.re, -fm1.im), djscaled_d(xx) / S);
       ddj = scale(fm1, ddjscaled_d(xx) / (S * S));
     }
+    dip[t] = f;
     const double d = j.re * j.re + j.im * j.im;
     const z64 a = mk(j.re / d, -j.im / d);  // 1/J
     const z64 a2 = a * a, a3 = a2 * a, a4 = a2 * a2;
@@ -234,6 +252,7 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
       if (hipMalloc(&c->invj, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
       if (rc == SDV_OK && hipMalloc(&c->sdiag, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
     }
+    if (rc == SDV_OK && hipMalloc(&c->dip, n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
     if (rc == SDV_OK && cfg->maskflag && !cfg->scalingflag) {  // maskfunction, SD/init_sinc_new.f90:580-599
       std::vector<double> m(n, 1.0);
       const int k = cfg->masknumpoints;
@@ -249,7 +268,7 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
     if (rc == SDV_OK) {
       hipLaunchKernelGGL(k_init_tables, dim3((2 * n + 255) / 256), dim3(256), 0, c->stream, n, cfg->spacing,
                          cfg->scalingflag ? 1 : 0, cfg->scalingtheta, cfg->scalingstretch, cfg->scalingdistance,
-                         cfg->smoothness, c->ke, c->invj, c->sdiag);
+                         cfg->smoothness, c->ke, c->invj, c->sdiag, c->dip);
       if (hipStreamSynchronize(c->stream) != hipSuccess) rc = SDV_ERR_HIP;
     }
     if (rc == SDV_ERR_NOMEM) set_error("device allocation failed in sdv_init");
@@ -270,6 +289,7 @@ int sdv_free(sdv_ctx* c) {
   hipFree(c->invj);
   hipFree(c->sdiag);
   hipFree(c->mask);
+  hipFree(c->dip);
   for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2, &c->ob_ke,
                           &c->ob_scr, &c->h_pot, &c->ar_w, &c->ar_rp, &c->ar_s, &c->ar_s2, &c->ar_dots,
                           &c->h_small, &c->h_out3})
@@ -877,8 +897,9 @@ int sdv_pot_matel_dev(sdv_ctx* c, int low, int high, const void* inspfs1, const 
 }
 
 int sdv_actreduced00_dev(sdv_ctx* c, const void* inspfs, const void* projspfs, const void* denmat,
-                         const void* invdenmat, const void* reducedpot, const double* pot, int dentimeflag,
-                         int projflag, double timefac_re, double timefac_im, void* outspfs, void* stream) {
+                         const void* invdenmat, const void* reducedpot, const double* pot, const double* tdpot,
+                         int dentimeflag, int projflag, double timefac_re, double timefac_im, void* outspfs,
+                         void* stream) {
   SDV_REQUIRE(c && inspfs && denmat && reducedpot && outspfs, SDV_ERR_ARG, "null argument");
   SDV_REQUIRE(!dentimeflag || invdenmat, SDV_ERR_ARG, "dentimeflag needs invdenmat");
   SDV_REQUIRE(!projflag || projspfs, SDV_ERR_ARG, "projflag needs projspfs");
@@ -906,7 +927,8 @@ int sdv_actreduced00_dev(sdv_ctx* c, const void* inspfs, const void* projspfs, c
   a.out = c->ob_ke.p;
   SDV_TRY(ke_launch(a, c->precision, st, &c->prof));                                       // :691-693
   SDV_TRY(mult_reducedpot_launch(n3, N, N, inspfs, reducedpot, c->ar_rp.p, st));           // :762-764
-  SDV_TRY(orb_sum_launch(n3, N, c->ob_ke.p, pot, c->ar_w.p, c->ar_rp.p, c->ar_s.p, st));  // + mult_pot :698
+  SDV_TRY(orb_sum_launch(c->n, N, c->ob_ke.p, pot, c->ar_w.p, c->ar_rp.p, c->dip, tdpot, c->ar_s.p,
+                         st));  // + mult_pot :698, pulse :735-741
   void* cur = c->ar_s.p;
   if (dentimeflag) {  // spfmult . invdenmat * timefac   :780-787
     SDV_TRY(orb_gemm_launch(n3, N, N, cur, invdenmat, N, mk(timefac_re, timefac_im), zero, c->ar_s2.p, st));
@@ -1039,8 +1061,8 @@ int sdv_pot_matel(sdv_ctx* c, int low, int high, const double* inspfs1, const do
 }
 
 int sdv_actreduced00(sdv_ctx* c, const double* inspfs, const double* projspfs, const double* denmat,
-                     const double* invdenmat, const double* reducedpot, const double* pot, int dentimeflag,
-                     int projflag, double timefac_re, double timefac_im, double* outspfs) {
+                     const double* invdenmat, const double* reducedpot, const double* pot, const double* tdpot,
+                     int dentimeflag, int projflag, double timefac_re, double timefac_im, double* outspfs) {
   SDV_REQUIRE(c && inspfs && denmat && reducedpot && outspfs, SDV_ERR_ARG, "null argument");
   const size_t pb = (size_t)c->n3 * c->N * sizeof(z64), nn = (size_t)c->N * c->N * sizeof(z64);
   const void *din = nullptr, *dproj = nullptr, *drp = nullptr, *dpot = nullptr;
@@ -1054,8 +1076,8 @@ int sdv_actreduced00(sdv_ctx* c, const double* inspfs, const double* projspfs, c
   if (dentimeflag && invdenmat) SDV_HIP_CHECK(hipMemcpyAsync(dm + nn, invdenmat, nn, hipMemcpyHostToDevice, c->stream));
   SDV_TRY(ensure(c->h_out3, pb));
   SDV_TRY(sdv_actreduced00_dev(c, din, dproj, dm, (dentimeflag && invdenmat) ? dm + nn : nullptr, drp,
-                               (const double*)dpot, dentimeflag, projflag, timefac_re, timefac_im, c->h_out3.p,
-                               c->stream));
+                               (const double*)dpot, tdpot, dentimeflag, projflag, timefac_re, timefac_im,
+                               c->h_out3.p, c->stream));
   SDV_HIP_CHECK(hipMemcpyAsync(outspfs, c->h_out3.p, pb, hipMemcpyDeviceToHost, c->stream));
   SDV_HIP_CHECK(hipStreamSynchronize(c->stream));
   return SDV_OK;

@@ -36,6 +36,7 @@ struct sdv_ctx {
   double* ke = nullptr;
   sdv::z64* invj = nullptr;
   sdv::z64* sdiag = nullptr;
+  sdv::z64* dip = nullptr;  // dipole (coordinate, complex-scaled with ECS) per grid index, same on the three axes
   double* mask = nullptr;  // edge mask (n values, same on the three axes) when maskflag and not scalingflag
   hipStream_t stream = nullptr;
   // growable device buffers

@@ -173,11 +173,12 @@ int dot_launch(int n3, int na, int nb, const void* A, const void* B, const doubl
                void* out, int ld, hipStream_t st);
 
 // Orbital operator pieces: out = alpha A C + beta out (A: n3 x nin device
-// columns, C: nin x nout device, column-major ldc); s = ke + pot w + rp.
+// columns, C: nin x nout device, column-major ldc); s = ke + pot w + pulse
+// (tdpot: 3 complex field values x, y, z on the host, or null) + rp.
 int orb_gemm_launch(int n3, int nin, int nout, const void* A, const void* C, int ldc, z64 alpha, z64 beta, void* out,
                     hipStream_t st);
-int orb_sum_launch(int n3, int ncols, const void* ke, const double* pot, const void* w, const void* rp, void* s,
-                   hipStream_t st);
+int orb_sum_launch(int n, int ncols, const void* ke, const double* pot, const void* w, const void* rp,
+                   const void* dip, const double* tdpot, void* s, hipStream_t st);
 
 // Mean field
 int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, hipStream_t st);

@@ -156,22 +156,41 @@ int orb_gemm_launch(int n3, int nin, int nout, const void* A, const void* C, int
   return SDV_OK;
 }
 
-// s(x, j) = ke(x, j) + pot(x) w(x, j) + rp(x, j)   (actreduced00's spfmult sum)
-__global__ void k_orb_sum(size_t total, int n3, const z64* __restrict__ ke, const double* __restrict__ pot,
-                          const z64* __restrict__ w, const z64* __restrict__ rp, z64* __restrict__ s) {
+// s(x, j) = ((ke + pot w) + pulse) + rp, actreduced00's spfmult sum
+// (S/derivs.f90:691-764); pulse = length-gauge lenmultiply
+// (S/proputils.f90:157-182): ((w dz(z)) tz + (w dy(y)) ty) + (w dx(x)) tx
+// with the per-axis dipoles dip (the grid coordinate, complex-scaled under ECS);
+// an axis whose field value is zero is skipped (bit 0 x, 1 y, 2 z of axes).
+__global__ void k_orb_sum(size_t total, int n, const z64* __restrict__ ke, const double* __restrict__ pot,
+                          const z64* __restrict__ w, const z64* __restrict__ rp, const z64* __restrict__ dip, z64 tx,
+                          z64 ty, z64 tz, int axes, z64* __restrict__ s) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= total) return;
+  const size_t n3 = (size_t)n * n * n, g = e % n3;
   z64 v = ke[e];
-  if (pot) v = v + scale(w[e], pot[e % n3]);
+  if (pot) v = v + scale(w[e], pot[g]);
+  if (axes) {
+    const z64 we = w[e];
+    z64 p = mk(0.0, 0.0);
+    if (axes & 4) p = p + (we * dip[g / ((size_t)n * n)]) * tz;
+    if (axes & 2) p = p + (we * dip[(g / n) % n]) * ty;
+    if (axes & 1) p = p + (we * dip[g % n]) * tx;
+    v = v + p;
+  }
   s[e] = v + rp[e];
 }
 
-int orb_sum_launch(int n3, int ncols, const void* ke, const double* pot, const void* w, const void* rp, void* s,
-                   hipStream_t st) {
-  const size_t total = (size_t)n3 * ncols;
+int orb_sum_launch(int n, int ncols, const void* ke, const double* pot, const void* w, const void* rp,
+                   const void* dip, const double* tdpot, void* s, hipStream_t st) {
+  const size_t total = (size_t)n * n * n * ncols;
   if (!total) return SDV_OK;
-  hipLaunchKernelGGL(k_orb_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, n3, (const z64*)ke,
-                     pot, (const z64*)w, (const z64*)rp, (z64*)s);
+  const z64 zero = mk(0.0, 0.0);
+  const z64 tx = tdpot ? mk(tdpot[0], tdpot[1]) : zero, ty = tdpot ? mk(tdpot[2], tdpot[3]) : zero,
+            tz = tdpot ? mk(tdpot[4], tdpot[5]) : zero;
+  auto on = [](z64 t) { return (t.re != 0.0 || t.im != 0.0) ? 1 : 0; };
+  const int axes = on(tx) | (on(ty) << 1) | (on(tz) << 2);
+  hipLaunchKernelGGL(k_orb_sum, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, total, n, (const z64*)ke,
+                     pot, (const z64*)w, (const z64*)rp, (const z64*)dip, tx, ty, tz, axes, (z64*)s);
   SDV_HIP_CHECK(hipGetLastError());
   return SDV_OK;
 }

@@ -113,8 +113,8 @@ def load():
     lib.sdv_sparseops_matel_dev.argtypes = [vp, i, i, vp, vp, vp, vp]
     lib.sdv_pot_matel.argtypes = [vp, i, i, vp, vp, vp, vp]
     lib.sdv_pot_matel_dev.argtypes = [vp, i, i, vp, vp, vp, vp, vp]
-    lib.sdv_actreduced00.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, d, d, vp]
-    lib.sdv_actreduced00_dev.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, d, d, vp, vp]
+    lib.sdv_actreduced00.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i, i, d, d, vp]
+    lib.sdv_actreduced00_dev.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i, i, d, d, vp, vp]
     lib.sdv_circulant_sizes.argtypes = [vp, i]
     lib.sdv_set_circulant.argtypes = [vp, i]
     lib.sdv_set_pair_batch.argtypes = [vp, i]

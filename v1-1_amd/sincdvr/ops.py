@@ -264,13 +264,18 @@ class SincDVR:
         return out
 
     # ------------------------------------------------ orbital operator apply
-    def actreduced00(self, inspfs, denmat, reducedpot, pot=None, invdenmat=None, timefac=None, projspfs=None):
+    def actreduced00(self, inspfs, denmat, reducedpot, pot=None, invdenmat=None, timefac=None, projspfs=None,
+                     tdpot=None):
         """S/derivs.f90:629-852 over all orbitals (see sdv_actreduced00):
-        out = (1 - P)[timefac (T w + pot w + mult_reducedpot) invdenmat],
+        out = (1 - P)[timefac (T w + pot w + pulse w + mult_reducedpot) invdenmat],
         w = inspfs . denmat; P projects on projspfs (flavour dot).  The
         dentime step runs when invdenmat is given, the projection when
-        projspfs is given.  numpy (Fortran layout) or torch (reversed shape)."""
+        projspfs is given; tdpot = (Ex, Ey, Ez) adds the length-gauge pulse.
+        numpy (Fortran layout) or torch (reversed shape)."""
         tf = complex(1.0 if timefac is None else timefac)
+        td = None
+        if tdpot is not None:
+            td = (ctypes.c_double * 6)(*[v for c in tdpot for v in (complex(c).real, complex(c).imag)])
         dent = invdenmat is not None
         proj = projspfs is not None
         if _is_torch(inspfs):
@@ -279,7 +284,7 @@ class SincDVR:
             out = torch.empty_like(p)
             args = [_dptr(p), _dptr(projspfs.contiguous()) if proj else None, _dptr(dm),
                     _dptr(invdenmat.contiguous()) if dent else None, _dptr(reducedpot.contiguous()),
-                    ctypes.c_void_p(pot.contiguous().data_ptr()) if pot is not None else None]
+                    ctypes.c_void_p(pot.contiguous().data_ptr()) if pot is not None else None, td]
             _lib.check(self._lib.sdv_actreduced00_dev(self._h, *args, int(dent), int(proj), tf.real, tf.imag,
                                                       _dptr(out), _stream()))
             return out
@@ -292,7 +297,7 @@ class SincDVR:
         out = np.zeros_like(p, order="F")
         _lib.check(self._lib.sdv_actreduced00(self._h, _hptr(p), _hptr(pj) if proj else None, _hptr(dm),
                                               _hptr(idm) if dent else None, _hptr(rp),
-                                              ctypes.c_void_p(pt.ctypes.data) if pt is not None else None,
+                                              ctypes.c_void_p(pt.ctypes.data) if pt is not None else None, td,
                                               int(dent), int(proj), tf.real, tf.imag, _hptr(out)))
         return out
 
