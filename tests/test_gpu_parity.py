@@ -181,6 +181,31 @@ def test_twoe_matel_c1_41(orc, torch_dev):
     assert relerr(tm, otm) < TOL64
 
 
+@pytest.mark.parametrize("cnorm", [False, True])
+def test_twoe_matel_mixed_charge_carrier(orc, torch_dev, cnorm):
+    """Mixed form (FP32 butterflies) at 41^3 with 10 orbitals: the two-electron
+    matrix elements stay within 1e-7 (max-abs error over the largest element)
+    of the oracle.  When bra and ket are the same array, each diagonal pair
+    density convolves as rho_aa - g in FP32 with K*g (g a unit-charge Gaussian)
+    added in FP64; with distinct arrays of equal values the carrier is off and
+    the plain FP32 error, which the carrier must beat, remains."""
+    from sincdvr import SincDVR, inputs
+    cfg = inputs.CONFIGS["c1_41"]
+    n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
+    phi = inputs.gaussian_orbitals(n, dx, N, cfg["ngauss"], cfg["seed"], cnorm=cnorm)
+    kern = orc.coulomb_kernel(n, dx)
+    otm, otr = orc.twoe_matel(n, kern, phi, phi, M=81, cnorm=cnorm)
+    s = SincDVR(n, dx, N, cnorm=cnorm, precision=1)
+    tm, tr = s.call_twoe_matel00(1, N, phi, phi)
+    tm2, _ = s.call_twoe_matel00(1, N, phi, phi.copy())
+    e_carrier, e_plain = relerr(tm, otm), relerr(tm2, otm)
+    say("mixed n=41 N=10 cnorm=%d: twoematel err %.3e with carrier, %.3e without; twoereduced %.3e"
+        % (cnorm, e_carrier, e_plain, relerr(tr, otr)))
+    assert e_carrier < 1e-7
+    assert e_carrier < e_plain
+    assert relerr(tr, otr) < 2e-6
+
+
 def test_twoereduced_entry_and_profiling(torch_dev, c0):
     """sdv_twoereduced_dev delivers the same pair potentials as
     call_twoe_matel00, and the event profiler counts one launch per stage."""

@@ -68,6 +68,8 @@ struct KArgs {
   const cplx<T>* tw;
   const T* khat;
   int lds_tw2, lds_y;  // runtime LDS offsets (elements) of the k_yz twiddle table and plane
+  const double* gsub;  // FP32 form, bra = ket: charge carrier g and K*g of the spectrum (else nullptr)
+  const double* vsub;
 };
 
 // Everything one work item (one pair's convolution) needs, resolved from the
@@ -84,6 +86,8 @@ struct ItemView {
   const cplx<T>* tw;
   const T* khat;
   int lds_tw2, lds_y;
+  const double* gsub;  // diagonal pair of the FP32 form: rho - g is convolved, K*g added to the output
+  const double* vsub;
 };
 
 template <typename T>
@@ -105,6 +109,9 @@ __device__ __forceinline__ ItemView<T> make_view(const KArgs<T>& a, int pidx, in
   v.khat = a.khat;
   v.lds_tw2 = a.lds_tw2;
   v.lds_y = a.lds_y;
+  const bool sub = a.gsub && !a.density_mode && it.a == it.b;
+  v.gsub = sub ? a.gsub : nullptr;
+  v.vsub = sub ? a.vsub : nullptr;
   return v;
 }
 
@@ -168,12 +175,16 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
   constexpr int ITER = (LA * R + XCfg<P, Q>::XT - 1) / XCfg<P, Q>::XT;  // n <= R
   const int ne = nl * n;
   z64 ua[ITER], ub[ITER];
+  double gs[ITER];
+  const double* gsub = a.gsub ? a.gsub + (size_t)l0 * n : nullptr;
 #pragma unroll
   for (int i = 0; i < ITER; ++i) {
     const int e = tid + i * XCfg<P, Q>::XT;
+    gs[i] = 0.0;
     if (e < ne) {
       ua[i] = pa[e];
       if (!a.density_mode) ub[i] = pb[e];
+      if (gsub) gs[i] = gsub[e];
     }
   }
   RowCol rc(tid, XCfg<P, Q>::XT, n);
@@ -186,6 +197,7 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
         if (a.conj_b) w.im = -w.im;
         v = v * w;
       }
+      v.re -= gs[i];  // neutral remainder of a diagonal pair density (FP32 form)
       buf[rc.row * pitch + rc.col] = cvt<T>(v);
     }
   }
@@ -533,10 +545,12 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
   // twoereduced(:, col) and the mirror column (SD/coreproject.f90:521,528)
   z64* o = a.out + (size_t)l0 * n;
   z64* om = a.outm ? a.outm + (size_t)l0 * n : nullptr;
+  const double* vsub = a.vsub ? a.vsub + (size_t)l0 * n : nullptr;
   RowCol rc(tid, XCfg<P, Q>::XT, n);
   for (int e = tid; e < nl * n; e += XCfg<P, Q>::XT, rc.next()) {
     const cplx<T> v = buf[rc.row * pitch + rc.col];
     z64 d = mk((double)v.re, (double)v.im);
+    if (vsub) d.re += vsub[e];  // K*g of the charge carrier, in FP64
     o[e] = d;
     if (om) {
       if (a.mirror_conj) d.im = -d.im;
@@ -570,6 +584,9 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   a.out = (z64*)h.out;
   a.tw = (const cplx<T>*)(sizeof(T) == 8 ? s.tw64 : s.tw32);
   a.khat = (const T*)(sizeof(T) == 8 ? (const void*)s.khat64 : (const void*)s.khat32);
+  const bool carrier = sizeof(T) == 4 && !h.density_mode && h.phi1 == h.phi2 && s.gsub && s.vsub;
+  a.gsub = carrier ? s.gsub : nullptr;
+  a.vsub = carrier ? s.vsub : nullptr;
   const size_t shX = (size_t)(M + LA * pitch) * sizeof(cplx<T>);
   a.lds_tw2 = M;
   a.lds_y = M + R + ((M + R) & 1);  // keep the workspaces 16-byte aligned for FP32
@@ -607,6 +624,8 @@ struct BSetDev {
   z64* out;
   const cplx<T>* tw;
   const T* khat;
+  const double* gsub;  // FP32 form: charge carrier (bra = ket always holds in a batch)
+  const double* vsub;
   int n, m, conj_b, mirror_conj;
 };
 
@@ -629,6 +648,9 @@ __device__ __forceinline__ ItemView<T> batch_view(const BSetDev<T>& s, const BIt
   const int R = s.m, M = 2 * R;
   v.lds_tw2 = M;
   v.lds_y = M + R + ((M + R) & 1);
+  const bool sub = s.gsub && it.a == it.b;
+  v.gsub = sub ? s.gsub : nullptr;
+  v.vsub = sub ? s.vsub : nullptr;
   return v;
 }
 
@@ -972,6 +994,7 @@ void batch_fill_set(int precision, const BatchSetHost& s, void* dst) {
     b.out = (z64*)s.out;
     b.tw = (const z64*)s.spec->tw64;
     b.khat = s.spec->khat64;
+    b.gsub = b.vsub = nullptr;
     b.n = s.spec->g.n;
     b.m = s.spec->g.m;
     b.conj_b = cn ? 0 : 1;
@@ -983,6 +1006,8 @@ void batch_fill_set(int precision, const BatchSetHost& s, void* dst) {
     b.out = (z64*)s.out;
     b.tw = (const c32*)s.spec->tw32;
     b.khat = s.spec->khat32;
+    b.gsub = s.spec->gsub;
+    b.vsub = s.spec->vsub;
     b.n = s.spec->g.n;
     b.m = s.spec->g.m;
     b.conj_b = cn ? 0 : 1;

@@ -55,6 +55,13 @@ struct Spectrum {
   double* khat64 = nullptr; // (m+1)^3, [kkx][kky][kkz], includes 1/M^3
   float* khat32 = nullptr;
   double* koct = nullptr;   // n^3 octant of the real-space kernel K(i,j,k), i,j,k >= 0
+  // Charge carrier of the FP32 form: a unit-charge Gaussian g on the grid
+  // (n^3, real) and its potential K*g computed once in FP64.  The FP32
+  // pipeline convolves rho_aa - g and adds K*g back in FP64, so the large
+  // low-frequency part of a diagonal pair density never passes through FP32
+  // butterflies (the linear convolution is unchanged).
+  double* gsub = nullptr;
+  double* vsub = nullptr;
 };
 
 struct ConvArgsHost {

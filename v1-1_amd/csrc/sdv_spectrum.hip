@@ -21,6 +21,9 @@
 // cos(2pi k_x i/M) cos(2pi k_y j/M) cos(2pi k_z l/M) with w_0 = 1, w_i = 2.
 // It is evaluated as three separable cosine contractions on the octant and
 // stored for k in [0, m]^3 with the 1/M^3 of SD/toeplitz_subs.f90:578.
+#include <algorithm>
+#include <cmath>
+
 #include "sdv_internal.h"
 
 namespace sdv {
@@ -254,6 +257,69 @@ int nuclear_pot_launch(const Spectrum& s, double spacing, int ncent, const int* 
 
 static inline unsigned nblk(long n, int t) { return (unsigned)((n + t - 1) / t); }
 
+__global__ void k_real_part(const z64* __restrict__ in, double* __restrict__ out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i].re;
+}
+
+// The FP32 form's charge carrier (see Spectrum::gsub): g(x) = exp(-|x|^2 /
+// (2 w^2)) normalised to sum 1 over the grid, centred on the grid, w = max(0.7
+// bohr, 1.5 spacing); K*g through the FP64 pipeline.
+static int build_charge_carrier(Spectrum* s, double spacing, hipStream_t st) {
+  const int n = s->g.n;
+  const long n3 = (long)n * n * n;
+  const double w = std::max(0.7, 1.5 * spacing);
+  std::vector<double> g1(n), g(n3);
+  for (int i = 0; i < n; ++i) {
+    const double x = (i - 0.5 * (n - 1)) * spacing;  // grid points of SD/SineDVR.f90:16-20, centred
+    g1[i] = std::exp(-x * x / (2.0 * w * w));
+  }
+  double tot = 0.0;
+  for (long k = 0; k < n; ++k)
+    for (long j = 0; j < n; ++j)
+      for (long i = 0; i < n; ++i) tot += g[i + n * (j + n * k)] = g1[i] * g1[j] * g1[k];
+  std::vector<z64> gz(n3);
+  for (long i = 0; i < n3; ++i) {
+    g[i] /= tot;
+    gz[i] = mk(g[i], 0.0);
+  }
+  struct Tmp {
+    void *den = nullptr, *out = nullptr, *work = nullptr, *item = nullptr;
+    ~Tmp() {
+      for (void* p : {den, out, work, item}) (void)hipFree(p);
+    }
+  } t;
+  SDV_HIP_CHECK(hipMalloc(&s->gsub, n3 * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&s->vsub, n3 * sizeof(double)));
+  SDV_HIP_CHECK(hipMalloc(&t.den, n3 * sizeof(z64)));
+  SDV_HIP_CHECK(hipMalloc(&t.out, n3 * sizeof(z64)));
+  SDV_HIP_CHECK(hipMalloc(&t.work, conv_work_bytes(s->g, 1, 0)));
+  SDV_HIP_CHECK(hipMalloc(&t.item, sizeof(PairItem)));
+  const PairItem it{0, 0, 0, -1};
+  SDV_HIP_CHECK(hipMemcpyAsync(s->gsub, g.data(), n3 * sizeof(double), hipMemcpyHostToDevice, st));
+  SDV_HIP_CHECK(hipMemcpyAsync(t.den, gz.data(), n3 * sizeof(z64), hipMemcpyHostToDevice, st));
+  SDV_HIP_CHECK(hipMemcpyAsync(t.item, &it, sizeof(PairItem), hipMemcpyHostToDevice, st));
+  ConvArgsHost a;
+  a.n = n;
+  a.npairs = 1;
+  a.phi1 = a.phi2 = t.den;
+  a.density_mode = 1;
+  a.conj_b = 0;
+  a.mirror_conj = 0;
+  a.pairs_dev = (const PairItem*)t.item;
+  a.work = t.work;
+  a.out = t.out;
+  {
+    const int rc = conv_launch(*s, a, 0, st, nullptr);
+    if (rc != SDV_OK) return rc;
+  }
+  hipLaunchKernelGGL(k_real_part, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, (const z64*)t.out, s->vsub,
+                     n3);
+  SDV_HIP_CHECK(hipGetLastError());
+  SDV_HIP_CHECK(hipStreamSynchronize(st));  // the host vectors and temporaries end here
+  return SDV_OK;
+}
+
 int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hipStream_t st, int min_m) {
   SDV_REQUIRE(nbig > nsmall, SDV_ERR_ARG, "toothnbig must exceed toothnsmall (SD/tooth.f90:114-117)");
   SDV_REQUIRE(nbig >= n - 1, SDV_ERR_ARG, "toothnbig must be >= numpoints-1 (SD/sincproject.f90:151-153)");
@@ -309,7 +375,7 @@ int build_spectrum(int n, double spacing, int nbig, int nsmall, Spectrum* s, hip
   hipLaunchKernelGGL(k_twiddles, dim3(nblk(g.M, 256)), dim3(256), 0, st, (z64*)s->tw64, (c32*)s->tw32, g.M);
   SDV_HIP_CHECK(hipGetLastError());
   SDV_HIP_CHECK(hipStreamSynchronize(st));
-  return SDV_OK;
+  return build_charge_carrier(s, spacing, st);
 }
 
 void free_spectrum(Spectrum* s) {
@@ -318,6 +384,8 @@ void free_spectrum(Spectrum* s) {
   hipFree(s->khat64);
   hipFree(s->khat32);
   hipFree(s->koct);
+  hipFree(s->gsub);
+  hipFree(s->vsub);
   *s = Spectrum();
 }
 
