@@ -53,6 +53,9 @@ typedef struct sdv_config {
   int maskflag;           /* 1 = kinetic apply through the edge mask when scalingflag = 0
                              (SD/coreproject.f90:1355-1376, sinc_params.f90:73)          */
   int masknumpoints;      /* mask width in points per end (SD/init_sinc_new.f90:580-599) */
+  int velflag;            /* pulse gauge of sdv_actreduced00 (S/derivs.f90:742-750): 0 length
+                             (lenmultiply), else velocity (velmultiply,
+                             SD/coreproject.f90:2119-2134)                             */
 } sdv_config;
 
 /* Fill `cfg` with the reference defaults (SD/sinc_params.f90). */
@@ -141,14 +144,19 @@ int sdv_pot_matel_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs1
 
 /* The orbital operator of the propagator's Krylov matvec, actreduced00
  * (S/derivs.f90:629-852) for the sinc-DVR project (one bond point, no frozen
- * orbitals, scalar or constraint terms; the pulse in the length gauge), over
- * all numspf orbitals:
+ * orbitals, scalar or constraint terms), over all numspf orbitals:
  *   w       = inspfs . denmat                        (spfinvr = spfinvrsq = spfr)
  *   spfmult = mult_ke(w) + pot * w + pulse(w) + mult_reducedpot(inspfs, reducedpot)
- *             pulse = lenmultiply (S/proputils.f90:157-182): the field values
- *             tdpot = {x.re, x.im, y.re, y.im, z.re, z.im} (host, or null for
- *             no field) times the dipoles (grid coordinates, complex-scaled
- *             under ECS, SD/init_sinc_new.f90:527,619-645)
+ *             with the pulse values tdpot = {x.re, x.im, y.re, y.im, z.re, z.im}
+ *             (host, or null for no field; what vectdpot returns for the gauge):
+ *             velflag = 0: lenmultiply (S/proputils.f90:157-182), the field
+ *             values times the dipoles (grid coordinates, complex-scaled under
+ *             ECS, SD/init_sinc_new.f90:527,619-645);
+ *             velflag != 0: velmultiply (SD/coreproject.f90:2119-2134), the
+ *             first-derivative sinc matrix fdtot per axis (SD/SineDVR.f90:25-33)
+ *             times -i A_d, through mult_general option 2 (:1321-1379):
+ *             invsqrtjacobian on both sides under ECS (SD/init_sinc_new.f90:504),
+ *             the edge mask under maskflag
  *   dentimeflag: spfmult = timefac * spfmult . invdenmat
  *   projflag:    out = spfmult - projspfs . <projspfs | spfmult>   (project00,
  *                S/proputils.f90:10-60, flavour dot), else out = spfmult.

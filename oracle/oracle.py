@@ -305,12 +305,63 @@ def lenmultiply(n, w, dip, tdpot):
     return out
 
 
+def _axis_mask(w, mask, n, op):
+    """divide_mask / mult_mask of SINCDVR/coreproject.f90:1273-1319: x, then
+    y, then z coordinate's mask value, one at a time."""
+    H = w.shape[1]
+    t = np.asfortranarray(w).reshape((n, n, n, H), order="F").copy()
+    m = np.asarray(mask, dtype=np.float64)
+    for d in range(3):
+        shape = [1, 1, 1, 1]
+        shape[d] = n
+        t = op(t, m.reshape(shape))
+    return t.reshape((n ** 3, H), order="F")
+
+
+def velmultiply(n, w, avec, fd, invsqrtjac=None, mask=None):
+    """SINCDVR/coreproject.f90:2119-2134 -> mult_general option 2
+    (:1321-1379) -> mult_all0 case 2 (:2089-2092), in numpy:
+        out = sum_d c_d W_d (-F_d) W_d w,   c_d = -i A_d,  F_d(p, q) = fdtot(q - p)
+    along axis d, W_d = invsqrtjacobian(:, d) under ECS (SD/init_sinc_new.f90:504),
+    divide_mask before and mult_mask after under maskflag, identity otherwise;
+    an axis with A_d = 0 is skipped (:1339).  fd: the 2n-1 values fdtot(-(n-1)..n-1)
+    of sinedvr; w: (n^3, howmany), x fastest."""
+    w = np.asfortranarray(w, dtype=np.complex128)
+    if w.ndim == 1:
+        w = w.reshape(-1, 1, order="F")
+    H = w.shape[1]
+    F = np.array([[fd[q - p + n - 1] for q in range(n)] for p in range(n)])      # F(p, q)
+    out = np.zeros_like(w, order="F")
+    for d in range(3):
+        c = -1j * complex(avec[d])
+        if c == 0:
+            continue
+        if invsqrtjac is not None:
+            t = w * np.asarray(invsqrtjac)[:, d][:, None]
+        elif mask is not None:
+            t = _axis_mask(w, mask, n, np.divide)
+        else:
+            t = w
+        T = t.reshape((n, n, n, H), order="F")
+        T2 = np.moveaxis(np.tensordot(T, -F, axes=([d], [0])), -1, d)           # out(., q, .) = -sum_p in(., p, .) F(p, q)
+        t2 = np.asfortranarray(T2).reshape((n ** 3, H), order="F")
+        if invsqrtjac is not None:
+            out = out + (t2 * np.asarray(invsqrtjac)[:, d][:, None]) * c
+        elif mask is not None:
+            out = out + _axis_mask(t2, mask, n, np.multiply) * c
+        else:
+            out = out + t2 * c
+    return out
+
+
 def actreduced00(n, ke, inspfs, denmat, reducedpot, pot=None, invdenmat=None, timefac=1.0, projspfs=None,
-                 cnorm=False, invjac=None, scalediag=None, mask=None, tdpot=None, dip=None):
+                 cnorm=False, invjac=None, scalediag=None, mask=None, tdpot=None, dip=None, velflag=0, fd=None):
     """S/derivs.f90:629-852 for the sinc-DVR project (one bond point): the
     kinetic apply is the C restatement (with ECS or the edge mask), the rest
     numpy, in the reference's order.  tdpot = (Ex, Ey, Ez) with dip
-    (dipoles_1d) adds the length-gauge pulse (:740-746)."""
+    (dipoles_1d) adds the length-gauge pulse (:742-746); with velflag it is
+    (Ax, Ay, Az) and adds velmultiply (:747-750) with the derivative stencil
+    fd (sinedvr), sqrt(invjac) as invsqrtjacobian under ECS, or the mask."""
     phi = np.asfortranarray(inspfs, dtype=np.complex128)
     w = phi @ np.asarray(denmat)                                   # spfinvrsq = spfinvr = spfr
     if mask is not None:
@@ -319,7 +370,9 @@ def actreduced00(n, ke, inspfs, denmat, reducedpot, pot=None, invdenmat=None, ti
         s = mult_ke(n, ke, w, invjac, scalediag)
     if pot is not None:
         s = s + w * np.asarray(pot)[:, None]
-    if tdpot is not None:
+    if tdpot is not None and velflag:
+        s = s + velmultiply(n, w, tdpot, fd, None if invjac is None else np.sqrt(np.asarray(invjac)), mask)
+    elif tdpot is not None:
         s = s + lenmultiply(n, w, dip, tdpot)
     s = s + mult_reducedpot(1, phi.shape[1], phi, reducedpot)
     if invdenmat is not None:

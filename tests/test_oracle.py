@@ -437,3 +437,60 @@ def test_dipoles_and_lenmultiply(orc):
     a = orc.actreduced00(n, ke, phi, np.eye(1), zero_rp)
     b = orc.actreduced00(n, ke, phi, np.eye(1), zero_rp, tdpot=E1, dip=dipe)
     assert relerr(b - a, orc.lenmultiply(n, phi, dipe, E1)) < 1e-13
+
+
+def test_velmultiply_restatement(orc):
+    """Velocity-gauge pulse restatement (velmultiply, first-derivative sinc
+    matrix).  Closed form: for a Gaussian well inside the grid the axis-z term
+    is (-i A_z)(-d/dz) of it (spectral accuracy of the sinc derivative), and
+    each axis differentiates along its own coordinate only (an axis swap
+    fails).  Structure: -F is real antisymmetric, so with ECS weights W on both
+    sides the operator is antisymmetric under the bilinear (c-norm) product,
+    and without weights and with real A it is Hermitian; it is linear in A; a
+    function vanishing wherever J != 1 only receives the post-weight sqrt(1/J);
+    under the edge mask it is the similarity M . op . M^-1."""
+    n, d = 31, 0.3
+    _, fd, pts = orc.sinedvr(n, d)
+    X, Y, Z = np.meshgrid(pts, pts, pts, indexing="ij")
+    z0, al = 0.4, 1.2
+    gau = np.exp(-al * (X ** 2 + Y ** 2 + (Z - z0) ** 2))
+    f = gau.transpose(2, 1, 0).reshape(-1, 1) + 0j                       # x fastest
+    dz = (-2 * al * (Z - z0) * gau).transpose(2, 1, 0).reshape(-1, 1)
+    dx = (-2 * al * X * gau).transpose(2, 1, 0).reshape(-1, 1)
+    Az = 0.7 - 0.2j
+    got = orc.velmultiply(n, f, (0, 0, Az), fd)
+    assert relerr(got, (-1j * Az) * (-dz)) < 1e-8
+    assert relerr(orc.velmultiply(n, f, (0.5, 0, 0), fd), (-0.5j) * (-dx)) < 1e-8
+    assert relerr(got, (-1j * Az) * (-dx)) > 0.5
+    n, d = 7, 0.4
+    _, fd, pts = orc.sinedvr(n, d)
+    rng = np.random.default_rng(12)
+    u, v = [rng.standard_normal((n ** 3, 1)) + 1j * rng.standard_normal((n ** 3, 1)) for _ in range(2)]
+    A1, A2 = (0.3 - 0.1j, -0.2, 0.5j), (0.1, 0.4 + 0.2j, -0.3)
+    assert relerr(orc.velmultiply(n, u, [a + b for a, b in zip(A1, A2)], fd),
+                  orc.velmultiply(n, u, A1, fd) + orc.velmultiply(n, u, A2, fd)) < 1e-14
+    Ar = (0.3, -0.2, 0.5)
+    lhs = np.vdot(v, orc.velmultiply(n, u, Ar, fd))
+    assert abs(lhs - np.conj(np.vdot(u, orc.velmultiply(n, v, Ar, fd)))) < 1e-13 * abs(lhs)
+    ij, _ = orc.ecs_arrays(n, d, 0.4, 1.2, 0.5, 0.4)
+    sj = np.sqrt(ij)
+    a = np.sum(v * orc.velmultiply(n, u, A1, fd, invsqrtjac=sj))
+    b = np.sum(orc.velmultiply(n, v, A1, fd, invsqrtjac=sj) * u)
+    assert abs(a + b) < 1e-13 * abs(a)
+    Xs, Ys, Zs = np.meshgrid(pts, pts, pts, indexing="ij")
+    mask_in = ((np.abs(Xs) <= 0.5) & (np.abs(Ys) <= 0.5) & (np.abs(Zs) <= 0.5)).transpose(2, 1, 0).reshape(-1, 1)
+    w = np.where(mask_in, u, 0)
+    assert np.allclose(sj[mask_in[:, 0]], 1.0)
+    plain = orc.velmultiply(n, w, A1, fd)
+    post = sum(orc.velmultiply(n, w, tuple(A1[k] if k == axis else 0 for k in range(3)), fd) * sj[:, [axis]]
+               for axis in range(3))
+    assert relerr(orc.velmultiply(n, w, A1, fd, invsqrtjac=sj), post) < 1e-14
+    assert relerr(post, plain) > 1e-3                                     # the post-weight matters
+    m = orc.mask_function(n, 2)
+    Mv = np.einsum("i,j,k->kji", m, m, m).reshape(-1, 1)
+    assert relerr(orc.velmultiply(n, u, A1, fd, mask=m), Mv * orc.velmultiply(n, u / Mv, A1, fd)) < 1e-13
+    ke, _, _ = orc.sinedvr(n, d)
+    zero_rp = np.zeros((n ** 3, 1, 1), dtype=complex, order="F")
+    base = orc.actreduced00(n, ke, u, np.eye(1), zero_rp)
+    withA = orc.actreduced00(n, ke, u, np.eye(1), zero_rp, tdpot=A1, velflag=1, fd=fd)
+    assert relerr(withA - base, orc.velmultiply(n, u, A1, fd)) < 1e-13

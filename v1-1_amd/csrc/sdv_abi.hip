@@ -45,8 +45,21 @@ __device__ double ddjscaled_d(double x) {
 // dip(i) = dipole (coordinate) of grid index i on one axis: the grid point,
 // or the complex-scaled coordinate F(x) with ECS (get_dipoles,
 // SD/init_sinc_new.f90:527,619-645; F of :5-30).
+// Principal square root (branch cut on the negative real axis, as Fortran's
+// complex SQRT), by the half-angle formulas written to avoid cancellation.
+__device__ z64 csqrt_principal(z64 z) {
+  const double r = hypot(z.re, z.im);
+  if (r == 0.0) return mk(0.0, 0.0);
+  if (z.re >= 0.0) {
+    const double t = sqrt(0.5 * (r + z.re));
+    return mk(t, z.im / (2.0 * t));
+  }
+  const double t = sqrt(0.5 * (r - z.re));
+  return mk(fabs(z.im) / (2.0 * t), copysign(t, z.im));
+}
+
 __global__ void k_init_tables(int n, double spacing, int ecs, double theta, double stretch, double R0, double S,
-                              double* ke, z64* invj, z64* sdiag, z64* dip) {
+                              double* ke, double* fd, z64* invj, z64* invsqj, z64* sdiag, z64* dip) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < 2 * n - 1) {
     const int i = t - (n - 1);
@@ -56,6 +69,10 @@ __global__ void k_init_tables(int n, double spacing, int ecs, double theta, doub
     else
       k = 2.0 / ((double)i * (double)i) * ((i & 1) ? -1.0 : 1.0);
     ke[t] = k / 2.0 / (spacing * spacing);
+    // first derivative fdtot(i) = (-1)^i / i / spacing, 0 at i = 0 (SD/SineDVR.f90:25-33),
+    // stored negated: mult_all0 option 2 contracts with alpha = -1 (SD/coreproject.f90:2089-2092)
+    const double f = i == 0 ? 0.0 : 1.0 / (double)i * ((i & 1) ? -1.0 : 1.0);
+    fd[t] = -(f / spacing);
   }
   if (!ecs && t < n) {
     const double p1 = 1 * spacing, pn = n * spacing;
@@ -103,8 +120,10 @@ __global__ void k_init_tables(int n, double spacing, int ecs, double theta, doub
     const z64 a2 = a * a, a3 = a2 * a, a4 = a2 * a2;
     const z64 t1 = scale(a4 * (dj * dj), 3.0 / 8.0);
     const z64 t2 = scale(a3 * ddj, 1.0 / 4.0);
+    const z64 sa = csqrt_principal(a);  // invsqrtjacobian = sqrt(invjacobian)   SD/init_sinc_new.f90:504
     for (int d3 = 0; d3 < 3; ++d3) {
       invj[d3 * n + t] = a;
+      invsqj[d3 * n + t] = sa;
       sdiag[d3 * n + t] = t1 - t2;
     }
   }
@@ -248,8 +267,10 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
   if (rc == SDV_OK) {
     const int n = c->n;
     if (hipMalloc(&c->ke, (2 * n) * sizeof(double)) != hipSuccess) rc = SDV_ERR_NOMEM;
+    if (rc == SDV_OK && hipMalloc(&c->fd, (2 * n) * sizeof(double)) != hipSuccess) rc = SDV_ERR_NOMEM;
     if (rc == SDV_OK && cfg->scalingflag) {
       if (hipMalloc(&c->invj, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
+      if (rc == SDV_OK && hipMalloc(&c->invsqj, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
       if (rc == SDV_OK && hipMalloc(&c->sdiag, 3 * n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
     }
     if (rc == SDV_OK && hipMalloc(&c->dip, n * sizeof(z64)) != hipSuccess) rc = SDV_ERR_NOMEM;
@@ -268,7 +289,7 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
     if (rc == SDV_OK) {
       hipLaunchKernelGGL(k_init_tables, dim3((2 * n + 255) / 256), dim3(256), 0, c->stream, n, cfg->spacing,
                          cfg->scalingflag ? 1 : 0, cfg->scalingtheta, cfg->scalingstretch, cfg->scalingdistance,
-                         cfg->smoothness, c->ke, c->invj, c->sdiag, c->dip);
+                         cfg->smoothness, c->ke, c->fd, c->invj, c->invsqj, c->sdiag, c->dip);
       if (hipStreamSynchronize(c->stream) != hipSuccess) rc = SDV_ERR_HIP;
     }
     if (rc == SDV_ERR_NOMEM) set_error("device allocation failed in sdv_init");
@@ -286,7 +307,9 @@ int sdv_free(sdv_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   spectrum_release(c->spec);
   hipFree(c->ke);
+  hipFree(c->fd);
   hipFree(c->invj);
+  hipFree(c->invsqj);
   hipFree(c->sdiag);
   hipFree(c->mask);
   hipFree(c->dip);
@@ -926,9 +949,20 @@ int sdv_actreduced00_dev(sdv_ctx* c, const void* inspfs, const void* projspfs, c
   a.in = c->ar_w.p;
   a.out = c->ob_ke.p;
   SDV_TRY(ke_launch(a, c->precision, st, &c->prof));                                       // :691-693
+  const bool vel = tdpot && c->cfg.velflag;
+  if (vel) {  // velmultiply (:747-750): sum_d (-i A_d) W_d (-fdtot) W_d spfinvr, added onto the kinetic apply
+    KeArgsHost v = a;
+    v.ke = c->fd;
+    v.invj = c->invsqj;  // invsqrtjacobian under ECS (SD/coreproject.f90:1325-1331,1356-1362)
+    v.sdiag = nullptr;
+    for (int d = 0; d < 3; ++d) v.coef[d] = mk(tdpot[2 * d + 1], -tdpot[2 * d]);  // (0,-1) * tdpot_d   :2128-2130
+    v.use_coef = 1;
+    v.accumulate = 1;
+    SDV_TRY(ke_launch(v, c->precision, st, nullptr));
+  }
   SDV_TRY(mult_reducedpot_launch(n3, N, N, inspfs, reducedpot, c->ar_rp.p, st));           // :762-764
-  SDV_TRY(orb_sum_launch(c->n, N, c->ob_ke.p, pot, c->ar_w.p, c->ar_rp.p, c->dip, tdpot, c->ar_s.p,
-                         st));  // + mult_pot :698, pulse :735-741
+  SDV_TRY(orb_sum_launch(c->n, N, c->ob_ke.p, pot, c->ar_w.p, c->ar_rp.p, c->dip, vel ? nullptr : tdpot,
+                         c->ar_s.p, st));  // + mult_pot :698, length-gauge pulse :742-746
   void* cur = c->ar_s.p;
   if (dentimeflag) {  // spfmult . invdenmat * timefac   :780-787
     SDV_TRY(orb_gemm_launch(n3, N, N, cur, invdenmat, N, mk(timefac_re, timefac_im), zero, c->ar_s2.p, st));

@@ -34,7 +34,9 @@ struct sdv_ctx {
   int auto_m = 0;      // half-edge choose_geom picks for n (the default circulant)
   sdv::Spectrum* spec = nullptr;  // shared with every context of the same grid (spectrum cache)
   double* ke = nullptr;
+  double* fd = nullptr;        // first-derivative stencil, negated (velocity gauge)
   sdv::z64* invj = nullptr;
+  sdv::z64* invsqj = nullptr;  // sqrt(1/J) per axis (ECS; velocity gauge)
   sdv::z64* sdiag = nullptr;
   sdv::z64* dip = nullptr;  // dipole (coordinate, complex-scaled with ECS) per grid index, same on the three axes
   double* mask = nullptr;  // edge mask (n values, same on the three axes) when maskflag and not scalingflag

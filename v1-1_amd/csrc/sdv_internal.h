@@ -170,6 +170,11 @@ struct KeArgsHost {
   const double* mask = nullptr;  // n edge-mask values (device), or null: maskflag without ECS
   const void* in;            // z64 (n^3, howmany)
   void* out;
+  // mult_general option 2 (first derivative, velocity gauge): axis d's result
+  // times coef[d] (an axis with coef 0 is skipped), added into `out`.
+  z64 coef[3] = {mk(1.0, 0.0), mk(1.0, 0.0), mk(1.0, 0.0)};
+  int use_coef = 0;
+  int accumulate = 0;        // 1: every axis adds into `out` (0: axis 0 overwrites it)
 };
 int ke_launch(const KeArgsHost& a, int precision, hipStream_t st, Prof* prof);
 

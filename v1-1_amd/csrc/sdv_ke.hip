@@ -19,6 +19,10 @@
 namespace sdv {
 
 constexpr int kKeThreads = 256;
+// flags of the axis kernels: multiply each axis's result by its coefficient
+// (mult_general option 2, mycoefs); axis 0 adds into `out` instead of
+// overwriting it (the first derivative accumulates onto the kinetic apply).
+constexpr int kKeCoef = 1, kKeAccumulate = 2;
 
 // maskflag (scalingflag 0): divide_mask / mult_mask of SD/coreproject.f90:1273-1319
 // divide by, then multiply with, the edge mask of the x, y and z coordinate of
@@ -40,7 +44,8 @@ template <int AXIS>
 __global__ void __launch_bounds__(kKeThreads) k_ke_axis(int n, const double* __restrict__ ke,
                                                          const z64* __restrict__ invj, const z64* __restrict__ sdiag,
                                                          const double* __restrict__ mask,
-                                                         const z64* __restrict__ in, z64* __restrict__ out, int la) {
+                                                         const z64* __restrict__ in, z64* __restrict__ out, int la,
+                                                         z64 coef, int flags) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* st = reinterpret_cast<double*>(smem_raw);      // 2n-1 stencil
   z64* buf = reinterpret_cast<z64*>(st + ((2 * n - 1 + 1) & ~1));  // la * n
@@ -106,10 +111,11 @@ __global__ void __launch_bounds__(kKeThreads) k_ke_axis(int n, const double* __r
     z64 r = mk(re, im);
     if (ij) r = r * ij[q];  // temp2 * invjacobian(:,d)   :1366-1368
     if (mask) r = mask_multiply(r, mask, g, n);
-    if (AXIS == 0) {
+    if (flags & kKeCoef) r = r * coef;  // * mycoefs(jj)   :1368,1374
+    if (AXIS == 0 && !(flags & kKeAccumulate)) {
       z64 o = mk(0.0, 0.0);
       dst[g] = o + r;
-    } else if (AXIS == 1) {
+    } else if (AXIS != 2) {
       dst[g] = dst[g] + r;
     } else {
       z64 o = dst[g] + r;
@@ -162,7 +168,8 @@ template <typename T, int AXIS>
 __global__ void __launch_bounds__(kKeThreads) k_ke_mma(int n, const double* __restrict__ ke,
                                                         const z64* __restrict__ invj, const z64* __restrict__ sdiag,
                                                         const double* __restrict__ mask,
-                                                        const z64* __restrict__ in, z64* __restrict__ out) {
+                                                        const z64* __restrict__ in, z64* __restrict__ out, z64 coef,
+                                                        int flags) {
   typedef typename Mma<T>::v4 v4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int NP = (n + 3) & ~3, QT = (n + 15) / 16, pitch = NP + 1;
@@ -262,9 +269,10 @@ __global__ void __launch_bounds__(kKeThreads) k_ke_mma(int n, const double* __re
     z64 r = mk((double)rv.re, (double)rv.im);
     if (ij) r = r * ij[q];  // temp2 * invjacobian(:,d)   :1366-1368
     if (mask) r = mask_multiply(r, mask, g, n);  // mult_mask   :1373-1374
-    if (AXIS == 0) {
+    if (flags & kKeCoef) r = r * coef;  // * mycoefs(jj)   :1368,1374
+    if (AXIS == 0 && !(flags & kKeAccumulate)) {
       dst[g] = mk(0.0, 0.0) + r;
-    } else if (AXIS == 1) {
+    } else if (AXIS != 2) {
       dst[g] = dst[g] + r;
     } else {
       z64 o = dst[g] + r;
@@ -278,6 +286,12 @@ __global__ void __launch_bounds__(kKeThreads) k_ke_mma(int n, const double* __re
   }
 }
 
+// An axis with a zero coefficient is skipped, as the reference skips it
+// (SD/coreproject.f90:1339); only when the axes accumulate into `out`.
+static bool axis_on(const KeArgsHost& a, int d) {
+  return !(a.use_coef && a.accumulate && a.coef[d].re == 0.0 && a.coef[d].im == 0.0);
+}
+
 template <typename T>
 static int ke_launch_mma(const KeArgsHost& a, hipStream_t st) {
   const int n = a.n;
@@ -287,15 +301,22 @@ static int ke_launch_mma(const KeArgsHost& a, hipStream_t st) {
   SDV_REQUIRE(n <= 16 * kKeMaxQT, SDV_ERR_UNSUPPORTED, "grid edge too large for the MFMA kinetic tile");
   const z64* ij = (const z64*)a.invj;
   const z64* sd = (const z64*)a.sdiag;
-  const unsigned nb0 = (unsigned)((n3 / n + kKeLines - 1) / kKeLines);
-  hipLaunchKernelGGL((k_ke_mma<T, 0>), dim3(nb0, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
-                     a.mask, (const z64*)a.in, (z64*)a.out);
-  const unsigned nb1 = (unsigned)(((size_t)(n + kKeLines - 1) / kKeLines) * n);
-  hipLaunchKernelGGL((k_ke_mma<T, 1>), dim3(nb1, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
-                     a.mask, (const z64*)a.in, (z64*)a.out);
-  const unsigned nb2 = (unsigned)(((size_t)n * n + kKeLines - 1) / kKeLines);
-  hipLaunchKernelGGL((k_ke_mma<T, 2>), dim3(nb2, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
-                     a.mask, (const z64*)a.in, (z64*)a.out);
+  const int flags = (a.use_coef ? kKeCoef : 0) | (a.accumulate ? kKeAccumulate : 0);
+  if (axis_on(a, 0)) {
+    const unsigned nb0 = (unsigned)((n3 / n + kKeLines - 1) / kKeLines);
+    hipLaunchKernelGGL((k_ke_mma<T, 0>), dim3(nb0, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
+                       a.mask, (const z64*)a.in, (z64*)a.out, a.coef[0], flags);
+  }
+  if (axis_on(a, 1)) {
+    const unsigned nb1 = (unsigned)(((size_t)(n + kKeLines - 1) / kKeLines) * n);
+    hipLaunchKernelGGL((k_ke_mma<T, 1>), dim3(nb1, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
+                       a.mask, (const z64*)a.in, (z64*)a.out, a.coef[1], flags);
+  }
+  if (axis_on(a, 2)) {
+    const unsigned nb2 = (unsigned)(((size_t)n * n + kKeLines - 1) / kKeLines);
+    hipLaunchKernelGGL((k_ke_mma<T, 2>), dim3(nb2, 1, a.howmany), dim3(kKeThreads), sh, st, n, a.ke, ij, sd,
+                       a.mask, (const z64*)a.in, (z64*)a.out, a.coef[2], flags);
+  }
   SDV_HIP_CHECK(hipGetLastError());
   return SDV_OK;
 }
@@ -319,22 +340,23 @@ int ke_launch(const KeArgsHost& a, int precision, hipStream_t st, Prof* prof) {
   SDV_REQUIRE(sh0 <= 64 * 1024, SDV_ERR_UNSUPPORTED, "grid edge too large for the KE tile");
   const z64* ij = (const z64*)a.invj;
   const z64* sd = (const z64*)a.sdiag;
-  {
+  const int flags = (a.use_coef ? kKeCoef : 0) | (a.accumulate ? kKeAccumulate : 0);
+  if (axis_on(a, 0)) {
     const unsigned nb = (unsigned)((n3 / n + la0 - 1) / la0);
     hipLaunchKernelGGL((k_ke_axis<0>), dim3(nb, 1, a.howmany), dim3(kKeThreads), sh0, st, n, a.ke, ij, sd,
-                       a.mask, (const z64*)a.in, (z64*)a.out, la0);
+                       a.mask, (const z64*)a.in, (z64*)a.out, la0, a.coef[0], flags);
   }
-  {
+  if (axis_on(a, 1)) {
     const size_t s = n;
     const unsigned nb = (unsigned)(((s + la0 - 1) / la0) * (n3 / (s * n)));
     hipLaunchKernelGGL((k_ke_axis<1>), dim3(nb, 1, a.howmany), dim3(kKeThreads), sh0, st, n, a.ke, ij, sd,
-                       a.mask, (const z64*)a.in, (z64*)a.out, la0);
+                       a.mask, (const z64*)a.in, (z64*)a.out, la0, a.coef[1], flags);
   }
-  {
+  if (axis_on(a, 2)) {
     const size_t s = (size_t)n * n;
     const unsigned nb = (unsigned)(((s + la0 - 1) / la0) * (n3 / (s * n)));
     hipLaunchKernelGGL((k_ke_axis<2>), dim3(nb, 1, a.howmany), dim3(kKeThreads), sh0, st, n, a.ke, ij, sd,
-                       a.mask, (const z64*)a.in, (z64*)a.out, la0);
+                       a.mask, (const z64*)a.in, (z64*)a.out, la0, a.coef[2], flags);
   }
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_KE, e0, st);

@@ -25,7 +25,7 @@ module sincdvr_capi
      integer(c_int) :: toothnbig, toothnsmall, numspf, cnormflag, scalingflag
      real(c_double) :: scalingtheta, scalingstretch, scalingdistance, smoothness
      integer(c_int) :: notwoflag, debugflag, precision
-     integer(c_int) :: maskflag, masknumpoints
+     integer(c_int) :: maskflag, masknumpoints, velflag
   end type sdv_config
 
   ! maskflag / masknumpoints of sinc_params (SD/sinc_params.f90:73-74), set by

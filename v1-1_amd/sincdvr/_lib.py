@@ -54,6 +54,7 @@ class SdvConfig(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("maskflag", ctypes.c_int),
         ("masknumpoints", ctypes.c_int),
+        ("velflag", ctypes.c_int),
     ]
 
 

@@ -56,7 +56,7 @@ class SincDVR:
     stencil and ECS tables (the project state of myprojectmod / myparams)."""
 
     def __init__(self, numpoints, spacing, numspf, cnorm=False, scaling=None, toothnbig=240,
-                 toothnsmall=40, notwoflag=0, debugflag=0, precision=0, masknumpoints=None):
+                 toothnsmall=40, notwoflag=0, debugflag=0, precision=0, masknumpoints=None, velflag=0):
         lib = _lib.load()
         cfg = _lib.SdvConfig()
         lib.sdv_config_default(ctypes.byref(cfg))
@@ -69,6 +69,7 @@ class SincDVR:
         cfg.notwoflag = notwoflag
         cfg.debugflag = debugflag
         cfg.precision = precision
+        cfg.velflag = int(velflag)   # pulse gauge of actreduced00: 0 length, else velocity
         if masknumpoints is not None:  # maskflag: kinetic apply through the edge mask (without ECS)
             cfg.maskflag = 1
             cfg.masknumpoints = int(masknumpoints)
@@ -270,8 +271,9 @@ class SincDVR:
         out = (1 - P)[timefac (T w + pot w + pulse w + mult_reducedpot) invdenmat],
         w = inspfs . denmat; P projects on projspfs (flavour dot).  The
         dentime step runs when invdenmat is given, the projection when
-        projspfs is given; tdpot = (Ex, Ey, Ez) adds the length-gauge pulse.
-        numpy (Fortran layout) or torch (reversed shape)."""
+        projspfs is given; tdpot adds the pulse: the field (Ex, Ey, Ez) in the
+        length gauge (velflag 0), the vector potential (Ax, Ay, Az) in the
+        velocity gauge.  numpy (Fortran layout) or torch (reversed shape)."""
         tf = complex(1.0 if timefac is None else timefac)
         td = None
         if tdpot is not None:
