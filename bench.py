@@ -16,11 +16,13 @@ orbitals already resident in HBM.  `value` is the plain FP64 form; beside it:
     p50/p99 latency at 70% of each form's measured capacity;
   * two-electron matrix elements and the kinetic apply, timed separately.
 
-Roofline: algorithmic bytes per pair follow the pencil model
+Roofline: algorithmic bytes per convolution follow the pencil model
 B = s (2 n^3 + 4 M n^2 + 4 M^2 n) (s = 16 FP64, 8 FP32), split over the
 kernels by the passes each performs: k_xfwd = x-forward (R n^3, W M n^2),
 k_yz = y-forward + z (fused multiply) + y-inverse (R+W 2 M n^2 + 4 M^2 n),
-k_xinv = x-inverse (R M n^2, W n^3).
+k_xinv = x-inverse (R M n^2, W n^3).  The 136 pair potentials take 128
+convolutions: the 16 diagonal pair densities |phi_a|^2 are real and run two
+per complex convolution.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -269,8 +271,10 @@ def main():
     dom = max(stages, key=lambda k: prof64[k][0])
     ms_tot, launches = prof64[dom]
     avg_ms = ms_tot / launches
+    nconv = s.convolutions(1, N)                 # 16 diagonal pairs run two per convolution
     pairs_per_launch = npairs * args.steps / launches
-    alg = alg_bytes_per_pair(dom, n, M, 16) * pairs_per_launch
+    conv_per_launch = nconv * args.steps / launches
+    alg = alg_bytes_per_pair(dom, n, M, 16) * conv_per_launch
     achieved = alg / (avg_ms * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_k_yz_81.json")
@@ -278,8 +282,10 @@ def main():
         with open(tfile) as f:
             tj = json.load(f)
         traffic = tj.get("bytes_per_launch")
-        if traffic is not None and tj.get("pairs_per_launch"):
-            traffic = traffic * pairs_per_launch / tj["pairs_per_launch"]
+        if traffic is not None and tj.get("convolutions_per_launch"):
+            traffic = traffic * conv_per_launch / tj["convolutions_per_launch"]
+        else:
+            traffic = None                            # counters of another work split
     out = {
         "metric": METRIC,
         "value": npairs / t64,
@@ -299,11 +305,12 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_" + dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": alg, "pairs_per_launch": pairs_per_launch, "avg_launch_ms": avg_ms},
-        "path_hbm_frac": alg_bytes_per_pair("path", n, M, 16) * npairs / t64 / 1e9 / HBM_PEAK_GBS,
+                     "alg_bytes_per_launch": alg, "pairs_per_launch": pairs_per_launch,
+                     "convolutions_per_launch": conv_per_launch, "avg_launch_ms": avg_ms},
+        "path_hbm_frac": alg_bytes_per_pair("path", n, M, 16) * nconv / t64 / 1e9 / HBM_PEAK_GBS,
         "stage_ms_per_step_fp64": {k: prof64[k][0] / args.steps for k in stages},
         "mixed": {"value": npairs / t32, "ms_per_step": t32 * 1e3, "speedup_vs_fp64": t64 / t32,
-                  "path_hbm_frac": alg_bytes_per_pair("path", n, M, 8) * npairs / t32 / 1e9 / HBM_PEAK_GBS,
+                  "path_hbm_frac": alg_bytes_per_pair("path", n, M, 8) * nconv / t32 / 1e9 / HBM_PEAK_GBS,
                   "dtype": "f32 butterflies, f64 spectrum construction and reductions",
                   "twoematel_error_vs_fp64": err,
                   "stage_ms_per_step": {k: prof32[k][0] / args.steps for k in stages}},

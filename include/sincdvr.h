@@ -122,6 +122,14 @@ int sdv_twoereduced(sdv_ctx* ctx, int lowspf, int highspf, const double* inspfs1
 int sdv_twoereduced_dev(sdv_ctx* ctx, int lowspf, int highspf, const void* inspfs10, const void* inspfs20,
                         void* twoereduced, void* stream);
 
+/* Number of convolutions sdv_twoereduced / call_twoe_matel00 run for
+ * (lowspf, highspf) with one array as bra and ket (same = 1) or two (same =
+ * 0).  No counterpart in the reference, which convolves every ordered pair
+ * (SD/coreproject.f90:1074-1079); here each unordered pair once, and for
+ * hermnorm two diagonal pairs (real densities |phi_a|^2) share one complex
+ * convolution.  For throughput accounting. */
+int sdv_twoe_convolutions(sdv_ctx* ctx, int lowspf, int highspf, int same);
+
 /* One-electron matrix elements (S/matel.f90:241-293 sparseops_matel,
  * :113-169 pot_matel), the rows after mult_ke in the driver:
  *   opmatel(numspf, lowspf:highspf)  = inspfs1^H . mult_ke(inspfs2(:, lowspf:highspf))
@@ -214,6 +222,8 @@ int sdv_batch_create(int nsets, const sdv_batch_set* sets, int precision, sdv_ba
 int sdv_batch_set_schedule(sdv_batch* batch, int mode, int grid_mult, int nstreams);
 int sdv_batch_run(sdv_batch* batch, void* stream);
 int sdv_batch_capture(sdv_batch* batch, void* stream);
+/* npairs: pair potentials delivered per step (the unordered pairs of every
+ * set); work_bytes: the convolution work buffer. */
 int sdv_batch_stats(sdv_batch* batch, long long* npairs, long long* work_bytes);
 int sdv_batch_free(sdv_batch* batch);
 /* Per-stage timing of sdv_batch_run (stages XFWD, YZ, XINV), as sdv_profile /

@@ -206,6 +206,30 @@ def test_twoe_matel_mixed_charge_carrier(orc, torch_dev, cnorm):
     assert relerr(tr, otr) < 2e-6
 
 
+@pytest.mark.parametrize("cnorm", [False, True])
+def test_convolution_count_and_packed_diagonals(orc, torch_dev, cnorm):
+    """Each unordered pair is convolved once; for hermnorm two diagonal pairs
+    (real densities) share one convolution, the odd one out runs alone.  The
+    packed diagonal potentials are real and match the oracle."""
+    from sincdvr import SincDVR, inputs
+    n, dx = 9, 0.5
+    for N, low, high in [(5, 1, 5), (7, 3, 7)]:
+        s = SincDVR(n, dx, N, cnorm=cnorm)
+        nd = high - low + 1                                     # diagonal pairs in range
+        offdiag = N * nd - nd - (nd * (nd - 1)) // 2            # unordered off-diagonal pairs
+        want = offdiag + ((nd + 1) // 2 if not cnorm else nd)
+        assert s.convolutions(low, high, same=True) == want
+        assert s.convolutions(low, high, same=False) == N * nd
+        phi = inputs.gaussian_orbitals(n, dx, N, 2, seed=91, cnorm=cnorm)
+        say("packed diagonals N=%d low=%d high=%d cnorm=%d" % (N, low, high, cnorm))
+        tm, tr = s.call_twoe_matel00(low, high, phi, phi)
+        otm, otr = orc.twoe_matel(n, orc.coulomb_kernel(n, dx), phi, phi, low=low, high=high, cnorm=cnorm)
+        assert relerr(tr, otr) < TOL64 and relerr(tm, otm) < TOL64
+        if not cnorm:  # packed in order (low, low+1), (low+2, low+3), ...; an odd last one runs alone
+            for b in range(low, low + 2 * (nd // 2)):
+                assert np.all(tr[:, b - 1, b - low].imag == 0.0)
+
+
 def test_twoereduced_entry_and_profiling(torch_dev, c0):
     """sdv_twoereduced_dev delivers the same pair potentials as
     call_twoe_matel00, and the event profiler counts one launch per stage."""

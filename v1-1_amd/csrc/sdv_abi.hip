@@ -489,6 +489,26 @@ int sdv::twoe_plan(sdv_ctx* c, int low, int high, bool sym, IdxPlan** out) {
   }
   pl.cols.resize(pl.items.size());
   for (size_t i = 0; i < pl.items.size(); ++i) pl.cols[i] = pl.items[i].col;
+  if (sym && herm) {
+    // The densities of diagonal pairs are real (|phi_a|^2): two of them share
+    // one complex convolution.  The reduction's column list above stays per
+    // pair; only the convolution's work list changes.
+    std::vector<PairItem> conv;
+    std::vector<size_t> diag;
+    for (size_t i = 0; i < pl.items.size(); ++i) {
+      if (pl.items[i].a == pl.items[i].b)
+        diag.push_back(i);
+      else
+        conv.push_back(pl.items[i]);
+    }
+    for (size_t k = 0; k + 1 < diag.size(); k += 2) {
+      const PairItem& p = pl.items[diag[k]];
+      const PairItem& q = pl.items[diag[k + 1]];
+      conv.push_back(PairItem{p.a, q.a, p.col, q.col, 1});
+    }
+    if (diag.size() % 2) conv.push_back(pl.items[diag.back()]);
+    pl.items.swap(conv);
+  }
   SDV_TRY(upload_plan(pl));
   auto ins = c->plans.emplace(key, std::move(pl));
   *out = &ins.first->second;
@@ -579,6 +599,15 @@ int sdv_twoereduced_dev(sdv_ctx* c, int low, int high, const void* phi1, const v
   return run_pairs(c, *pl, phi1, phi2, 0, twoereduced, st);
 }
 
+int sdv_twoe_convolutions(sdv_ctx* c, int low, int high, int same) {
+  SDV_REQUIRE(c, SDV_ERR_ARG, "null context");
+  if (high < low) return 0;
+  SDV_TRY(check_twoe_args(c, low, high));
+  IdxPlan* pl = nullptr;
+  SDV_TRY(twoe_plan(c, low, high, same != 0, &pl));
+  return (int)pl->items.size();
+}
+
 int sdv_call_twoe_matel00_dev(sdv_ctx* c, int low, int high, const void* phi1, const void* phi2, void* twoematel,
                               void* twoereduced, void* stream) {
   SDV_REQUIRE(c && phi1 && phi2 && twoematel && twoereduced, SDV_ERR_ARG, "null argument");
@@ -662,6 +691,7 @@ static int prof_read(const Prof& p, double* ms, int* count) {
 struct sdv_batch {
   std::vector<sdv_ctx*> ctxs;
   std::vector<BatchSetHost> sets;
+  long long pairs = 0;  // pair potentials delivered per step (unordered pairs of every set)
   BatchPlan plan;
   int precision = 0;
   int mode = BATCH_GROUPED;
@@ -773,6 +803,7 @@ int sdv_batch_create(int nsets, const sdv_batch_set* sets, int precision, sdv_ba
     BatchSetHost h;
     h.spec = c->spec;
     h.items = &pl->items;
+    b->pairs += (long long)pl->cols.size();
     h.phi = sets[i].inspfs;
     h.out = sets[i].twoereduced;
     h.cnorm = c->cfg.cnormflag;
@@ -831,7 +862,7 @@ int sdv_batch_capture(sdv_batch* b, void* stream) {
 
 int sdv_batch_stats(sdv_batch* b, long long* npairs, long long* work_bytes) {
   SDV_REQUIRE(b, SDV_ERR_ARG, "null batch");
-  if (npairs) *npairs = (long long)b->plan.items.size();
+  if (npairs) *npairs = b->pairs;
   if (work_bytes) *work_bytes = (long long)b->plan.x_bytes;
   return SDV_OK;
 }

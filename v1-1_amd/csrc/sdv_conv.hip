@@ -80,6 +80,7 @@ struct ItemView {
   const z64* pa;       // phi1 column a (orbital mode) or density column (density mode)
   const z64* pb;       // phi2 column b
   int density_mode, conj_b, mirror_conj;
+  int pack;            // two real diagonal densities |phi_a|^2 + i |phi_b|^2 (outputs Re to out, Im to outm)
   cplx<T>* X;          // this pair's 2m planes of n^2
   z64* out;            // output column
   z64* outm;           // mirror output column or nullptr
@@ -109,7 +110,8 @@ __device__ __forceinline__ ItemView<T> make_view(const KArgs<T>& a, int pidx, in
   v.khat = a.khat;
   v.lds_tw2 = a.lds_tw2;
   v.lds_y = a.lds_y;
-  const bool sub = a.gsub && !a.density_mode && it.a == it.b;
+  v.pack = it.pack;
+  const bool sub = a.gsub && !a.density_mode && (it.a == it.b || it.pack);
   v.gsub = sub ? a.gsub : nullptr;
   v.vsub = sub ? a.vsub : nullptr;
   return v;
@@ -192,12 +194,18 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
   for (int i = 0; i < ITER; ++i, rc.next()) {
     if (tid + i * XCfg<P, Q>::XT < ne) {
       z64 v = ua[i];
-      if (!a.density_mode) {  // SD/coreproject.f90:1057-1065: inspfs10(:,a) * CONJUGATE(inspfs20(:,b))
-        z64 w = ub[i];
-        if (a.conj_b) w.im = -w.im;
-        v = v * w;
+      if (a.pack) {  // |phi_a|^2 - g + i (|phi_b|^2 - g)
+        const z64 w = ub[i];
+        const z64 da = v * mk(v.re, -v.im), db = w * mk(w.re, -w.im);
+        v = mk(da.re - gs[i], db.re - gs[i]);
+      } else {
+        if (!a.density_mode) {  // SD/coreproject.f90:1057-1065: inspfs10(:,a) * CONJUGATE(inspfs20(:,b))
+          z64 w = ub[i];
+          if (a.conj_b) w.im = -w.im;
+          v = v * w;
+        }
+        v.re -= gs[i];  // neutral remainder of a diagonal pair density (FP32 form)
       }
-      v.re -= gs[i];  // neutral remainder of a diagonal pair density (FP32 form)
       buf[rc.row * pitch + rc.col] = cvt<T>(v);
     }
   }
@@ -550,7 +558,15 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
   for (int e = tid; e < nl * n; e += XCfg<P, Q>::XT, rc.next()) {
     const cplx<T> v = buf[rc.row * pitch + rc.col];
     z64 d = mk((double)v.re, (double)v.im);
-    if (vsub) d.re += vsub[e];  // K*g of the charge carrier, in FP64
+    if (vsub) {  // K*g of the charge carrier, in FP64
+      d.re += vsub[e];
+      if (a.pack) d.im += vsub[e];
+    }
+    if (a.pack) {  // two diagonal pair potentials, real
+      o[e] = mk(d.re, 0.0);
+      om[e] = mk(d.im, 0.0);
+      continue;
+    }
     o[e] = d;
     if (om) {
       if (a.mirror_conj) d.im = -d.im;
@@ -648,7 +664,8 @@ __device__ __forceinline__ ItemView<T> batch_view(const BSetDev<T>& s, const BIt
   const int R = s.m, M = 2 * R;
   v.lds_tw2 = M;
   v.lds_y = M + R + ((M + R) & 1);
-  const bool sub = s.gsub && it.a == it.b;
+  v.pack = it.pack;
+  const bool sub = s.gsub && (it.a == it.b || it.pack);
   v.gsub = sub ? s.gsub : nullptr;
   v.vsub = sub ? s.vsub : nullptr;
   return v;
@@ -829,6 +846,7 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
       it.b = p.b;
       it.col = p.col;
       it.mcol = p.mcol;
+      it.pack = p.pack;
       it.xoff = xoff;
       const int ii = (int)bp.items.size();
       bp.items.push_back(it);

@@ -35,8 +35,12 @@ struct Prof {
 // gets K * rho where rho = phi1[:,a] * CONJUGATE(phi2[:,b]) (orbital mode) or
 // rho = den[:,a] (density mode).  mcol >= 0 also receives the mirror pair
 // (b,a): conj(V) for hermnorm, V for cnorm.
+// pack = 1 (hermnorm, bra = ket): two diagonal pairs (a,a) and (b,b), whose
+// densities |phi|^2 are real, in one convolution of rho_aa + i rho_bb; column
+// col receives Re V, column mcol Im V.
 struct PairItem {
   int a, b, col, mcol;
+  int pack = 0;
 };
 
 // Geometry of one circulant: grid edge n, half edge m = P*Q >= n, M = 2m.
@@ -92,7 +96,7 @@ struct BatchSetHost {
   int cnorm;
 };
 struct BItemDev {
-  int set, a, b, col, mcol;
+  int set, a, b, col, mcol, pack;
   long long xoff;  // element offset of this pair's planes in the batch buffer
 };
 struct BWork {

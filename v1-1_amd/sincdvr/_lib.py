@@ -20,7 +20,7 @@ EXPORTS = [
     "sdv_mult_ke", "sdv_mult_ke_dev",
     "sdv_mult_reducedpot", "sdv_mult_reducedpot_dev",
     "sdv_get_reducedpot", "sdv_get_reducedpot_dev",
-    "sdv_twoereduced", "sdv_twoereduced_dev", "sdv_profile", "sdv_profile_read", "sdv_nuclear_potential",
+    "sdv_twoereduced", "sdv_twoereduced_dev", "sdv_twoe_convolutions", "sdv_profile", "sdv_profile_read", "sdv_nuclear_potential",
     "sdv_get_kernel_octant", "sdv_get_ke_stencil", "sdv_circulant_edge",
     "sdv_sparseops_matel", "sdv_sparseops_matel_dev", "sdv_pot_matel", "sdv_pot_matel_dev",
     "sdv_actreduced00", "sdv_actreduced00_dev",
@@ -104,6 +104,7 @@ def load():
     lib.sdv_get_reducedpot_dev.argtypes = [vp, vp, vp, vp, vp]
     lib.sdv_twoereduced.argtypes = [vp, i, i, vp, vp, vp]
     lib.sdv_twoereduced_dev.argtypes = [vp, i, i, vp, vp, vp, vp]
+    lib.sdv_twoe_convolutions.argtypes = [vp, i, i, i]
     lib.sdv_profile.argtypes = [vp, i]
     lib.sdv_nuclear_potential.argtypes = [vp, i, vp, vp, i, vp]
     lib.sdv_profile_read.argtypes = [vp, vp, vp]

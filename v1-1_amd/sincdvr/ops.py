@@ -184,6 +184,11 @@ class SincDVR:
                                                    _hptr(tr)))
         return tm, tr
 
+    def convolutions(self, lowspf, highspf, same=True):
+        """Convolutions twoereduced / call_twoe_matel00 run for this call shape."""
+        r = self._lib.sdv_twoe_convolutions(self._h, lowspf, highspf, int(bool(same)))
+        return _lib.check(r) if r < 0 else r
+
     def twoereduced(self, lowspf, highspf, inspfs10, inspfs20=None, out=None):
         """Pair potentials only (the convolution half of call_twoe_matel00)."""
         N = self.numspf

@@ -29,6 +29,7 @@ if os.path.dirname(HERE) not in sys.path:
 
 COPY_BYTES = 1 << 30          # read and written once by the calibration copy
 PAIRS_PER_LAUNCH = 136        # every pair of the 16-orbital step in one launch (the default)
+CONV_PER_LAUNCH = 128         # its convolutions: 120 off-diagonal pairs + 16 diagonal ones two per convolution
 KERNELS = ("k_xfwd", "k_yz", "k_xinv")
 
 
@@ -43,6 +44,7 @@ def run(precision, steps):
         inputs.gaussian_orbitals(n, dx, N, cfg["ngauss"], cfg["seed"]).T)).cuda()
     s = SincDVR(n, dx, N, precision=precision)
     s.set_pair_batch(PAIRS_PER_LAUNCH)
+    assert s.convolutions(1, N) == CONV_PER_LAUNCH, s.convolutions(1, N)
     tr = torch.zeros((N, N, n ** 3), dtype=torch.complex128, device="cuda")
     for i in range(steps):
         s.twoereduced(1, N, phi, out=tr)
@@ -120,15 +122,15 @@ def summarize(out_dir, prof_dir, tag="pairpot_81_n16"):
                      "per dispatch; factors (bytes per counted KiB / 1024) fetch %.3f, write %.3f." % (
                          COPY_BYTES, fetch["copy"], write["copy"], fk, wk))
         lines.append("")
-        lines.append("| kernel (full %d-pair dispatch) | FETCH KiB | WRITE KiB | HBM bytes (calibrated) |"
-                     % PAIRS_PER_LAUNCH)
+        lines.append("| kernel (full dispatch: %d pair potentials, %d convolutions) | FETCH KiB | WRITE KiB | "
+                     "HBM bytes (calibrated) |" % (PAIRS_PER_LAUNCH, CONV_PER_LAUNCH))
         lines.append("|---|---:|---:|---:|")
         for k in sorted(set(fetch) & set(write)):
             if k == "copy":
                 continue
             b = (fetch[k] * fk + write[k] * wk) * 1024.0
             traffic[k] = {"fetch_kib": fetch[k], "write_kib": write[k], "bytes_per_launch": b,
-                          "pairs_per_launch": PAIRS_PER_LAUNCH}
+                          "pairs_per_launch": PAIRS_PER_LAUNCH, "convolutions_per_launch": CONV_PER_LAUNCH}
             lines.append("| %s | %.0f | %.0f | %.3e |" % (k, fetch[k], write[k], b))
     os.makedirs(prof_dir, exist_ok=True)
     with open(os.path.join(prof_dir, f"{tag}_profile_summary.md"), "w") as f:
