@@ -130,6 +130,23 @@ struct XCfg {
 // A group could take LPG lines one after the other (smaller workgroups, two
 // per CU for FP32's 54 KB of LDS at m = 81); measured at 81^3 that is 1.4x
 // slower in FP32 than one line per group at 12 waves, so LPG = 1.
+// The y/z kernel stages its x-frequency plane's slice of the spectrum,
+// (m+1)^2 reals, in LDS after the plane when the workgroup's LDS still fits
+// (FP64 at m = 81: 109 KB plane and tables + 53 KB slice): the z pass then
+// reads it from LDS in the middle of its column FFTs instead of waiting on L2.
+#ifndef SDV_YZ_KHAT_LDS
+#define SDV_YZ_KHAT_LDS 1
+#endif
+constexpr size_t yz_base_bytes(int R, int es) {
+  return (size_t)(3 * R + (R & 1) + R * (R | 1)) * es;  // tables (2R + R + pad) and plane, bytes
+}
+constexpr bool yz_khat_in_lds(int R, int es) {
+  return SDV_YZ_KHAT_LDS && yz_base_bytes(R, es) + (size_t)(R + 1) * (R + 1) * (es / 2) <= 160 * 1024;
+}
+constexpr size_t yz_lds_bytes(int R, int es) {
+  return yz_base_bytes(R, es) + (yz_khat_in_lds(R, es) ? (size_t)(R + 1) * (R + 1) * (es / 2) : 0);
+}
+
 template <typename T, int P, int Q>
 struct YZCfg {
   static constexpr int R = P * Q;
@@ -311,11 +328,23 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   const int cx = plane / R, posx = plane - cx * R;
   const int kx = 2 * (posx / Q + P * (posx % Q)) + cx;
   const int kkx = min(kx, M - kx);
-  const T* kh = a.khat + (size_t)kkx * (R + 1) * (R + 1);
+  constexpr bool KH_LDS = yz_khat_in_lds(R, (int)sizeof(cplx<T>));
+  const T* khg = a.khat + (size_t)kkx * (R + 1) * (R + 1);
+  T* khs = reinterpret_cast<T*>(Y + R * pitch);
+  const T* kh = KH_LDS ? khs : khg;
   for (int i = t; i < M; i += NT) tw[i] = a.tw[i];
   for (int i = t; i < R; i += NT) {
     const int aa = i / P, bb = i - aa * P;
     T2[i] = a.tw[(2 * aa * bb) % M];
+  }
+  if (KH_LDS) {  // every load in flight before the first store
+    constexpr int KI = ((R + 1) * (R + 1) + NT - 1) / NT;
+    T kv[KI];
+#pragma unroll
+    for (int i = 0; i < KI; ++i) kv[i] = t + i * NT < (R + 1) * (R + 1) ? khg[t + i * NT] : (T)0;
+#pragma unroll
+    for (int i = 0; i < KI; ++i)
+      if (t + i * NT < (R + 1) * (R + 1)) khs[t + i * NT] = kv[i];
   }
   cplx<T> o0[KEEP ? LPG : 1][P];  // cy = 0 output part (FP32)
 
@@ -606,8 +635,9 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   const size_t shX = (size_t)(M + LA * pitch) * sizeof(cplx<T>);
   a.lds_tw2 = M;
   a.lds_y = M + R + ((M + R) & 1);  // keep the workspaces 16-byte aligned for FP32
-  const size_t shB = (size_t)(a.lds_y + R * pitch) * sizeof(cplx<T>);
-  SDV_REQUIRE(shB <= 160 * 1024, SDV_ERR_UNSUPPORTED, "y/z plane does not fit in LDS");
+  const size_t shB = yz_lds_bytes(R, (int)sizeof(cplx<T>));
+  SDV_REQUIRE(shB <= 160 * 1024 && shB >= (size_t)(a.lds_y + R * pitch) * sizeof(cplx<T>), SDV_ERR_UNSUPPORTED,
+              "y/z plane does not fit in LDS");
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
   SDV_REQUIRE(shX <= 160 * 1024, SDV_ERR_UNSUPPORTED, "x lines do not fit in LDS");
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xfwd<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shX));
@@ -808,9 +838,8 @@ __global__ void __launch_bounds__(kXThreads) k_xinv_grp(const BSetDev<T>* __rest
 // LDS of the x kernels and of the y/z kernel for half-edge m (elements of
 // cplx<T>).
 static size_t lds_x_elems(int m) { return (size_t)2 * m + (size_t)lines_per_block(m) * (m | 1); }
-static size_t lds_yz_elems(int m) {
-  const size_t R = m, M = 2 * R, pitch = R | 1;
-  return (M + R + ((M + R) & 1)) + R * pitch;
+static size_t lds_yz_elems(int m, int elem_bytes) {
+  return (yz_lds_bytes(m, elem_bytes) + elem_bytes - 1) / elem_bytes;
 }
 
 BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int mode, int grid_mult, int ncu) {
@@ -837,7 +866,7 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
       return bp;
     }
     shx = std::max(shx, lds_x_elems(m) * es);
-    shy = std::max(shy, lds_yz_elems(m) * es);
+    shy = std::max(shy, lds_yz_elems(m, (int)es) * es);
     nty = std::max(nty, yz_threads(m, precision));
     for (const PairItem& p : *s.items) {
       BItemDev it;
@@ -884,7 +913,7 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
       while (i < wy.size() && wy[i].m == g.m) ++i;
       g.nwy = (int)i - g.wy0;
       g.shx = lds_x_elems(g.m) * es;
-      g.shy = lds_yz_elems(g.m) * es;
+      g.shy = lds_yz_elems(g.m, (int)es) * es;
       bp.groups.push_back(g);
     }
     size_t gi = 0;
