@@ -137,6 +137,9 @@ struct XCfg {
 #ifndef SDV_YZ_KHAT_LDS
 #define SDV_YZ_KHAT_LDS 1
 #endif
+#ifndef SDV_YZ_TW_REGS
+#define SDV_YZ_TW_REGS 1
+#endif
 constexpr size_t yz_base_bytes(int R, int es) {
   return (size_t)(3 * R + (R & 1) + R * (R | 1)) * es;  // tables (2R + R + pad) and plane, bytes
 }
@@ -348,6 +351,17 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   }
   cplx<T> o0[KEEP ? LPG : 1][P];  // cy = 0 output part (FP32)
 
+  // Odd-channel twiddle w_M^(j1 Q + li) = w_M^li * w_2P^j1.  FP64: the lane's
+  // w_M^li (one LDS read) times the compile-time w_2P^j1, which saves P - 1
+  // 16-byte LDS reads per use at one complex multiply each; FP32 reads the
+  // correctly rounded table entry.
+  constexpr bool TW_REG = sizeof(T) == 8 && SDV_YZ_TW_REGS;
+  auto odd_base = [&]() -> cplx<T> { return TW_REG ? tw[li] : mk((T)0, (T)0); };
+  auto odd_tw = [&](const cplx<T>& wb, int j1) -> cplx<T> {
+    if (TW_REG) return j1 == 0 ? wb : wb * mk((T)kTw<2 * P>.c[j1], (T)-kTw<2 * P>.s[j1]);
+    return tw[j1 * Q + li];
+  };
+
   // row input x[j1] = plane(z = g, y = j1 Q + li)
   auto load_row = [&](int g, cplx<T>* x) {
     if (g < n && li < Q) {
@@ -365,10 +379,11 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     cplx<T>* yrow = Y + g * SR;
     if (rowok && li < Q) {
       cplx<T> v[P];
+      const cplx<T> wb = odd_base();
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
         const int y = j1 * Q + li;
-        v[j1] = (cy && y < n) ? x[j1] * tw[y] : x[j1];
+        v[j1] = (cy && y < n) ? x[j1] * odd_tw(wb, j1) : x[j1];
       }
       DFT<T, P, false>::run(v);
 #pragma unroll
@@ -404,8 +419,9 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     for (int cz = 0; cz < 2; ++cz) {
       if (li < Q) {
         cplx<T> v[P];
+        const cplx<T> wb = odd_base();
 #pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) v[j1] = cz ? u[j1] * tw[j1 * Q + li] : u[j1];
+        for (int j1 = 0; j1 < P; ++j1) v[j1] = cz ? u[j1] * odd_tw(wb, j1) : u[j1];
         DFT<T, P, false>::run(v);
 #pragma unroll
         for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * T2[li * P + k1];
@@ -431,8 +447,9 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 #pragma unroll
         for (int k1 = 0; k1 < P; ++k1) v[k1] = ycol[(k1 * Q + li) * SR];
         DFT<T, P, true>::run(v);
+        const cplx<T> wb = odd_base();
 #pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) acc[j1] = acc[j1] + (cz ? v[j1] * conj(tw[j1 * Q + li]) : v[j1]);
+        for (int j1 = 0; j1 < P; ++j1) acc[j1] = acc[j1] + (cz ? v[j1] * conj(odd_tw(wb, j1)) : v[j1]);
       }
     }
     if (li < Q) {
@@ -461,8 +478,9 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 #pragma unroll
       for (int k1 = 0; k1 < P; ++k1) v[k1] = yrow[(k1 * Q + li) * SP];
       DFT<T, P, true>::run(v);
+      const cplx<T> wb = odd_base();
 #pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) o[j1] = cy ? v[j1] * conj(tw[j1 * Q + li]) : v[j1];
+      for (int j1 = 0; j1 < P; ++j1) o[j1] = cy ? v[j1] * conj(odd_tw(wb, j1)) : v[j1];
     }
   };
 
