@@ -23,6 +23,7 @@
 // kept at position k1 Q + k2 and the inverse consumes that order directly.
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "sdv_dft.h"
 #include "sdv_internal.h"
@@ -415,8 +416,9 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         acc[j1] = mk((T)0, (T)0);
       }
     }
-#pragma unroll 1
-    for (int cz = 0; cz < 2; ++cz) {
+    // one z channel: forward, spectral multiply, inverse, accumulated
+    auto zchan = [&](auto czv) {
+      const int cz = czv;
       if (li < Q) {
         cplx<T> v[P];
         const cplx<T> wb = odd_base();
@@ -451,6 +453,13 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 #pragma unroll
         for (int j1 = 0; j1 < P; ++j1) acc[j1] = acc[j1] + (cz ? v[j1] * conj(odd_tw(wb, j1)) : v[j1]);
       }
+    };
+    if constexpr (sizeof(T) == 4) {  // FP32 has the registers for both channels unrolled
+      zchan(std::integral_constant<int, 0>{});
+      zchan(std::integral_constant<int, 1>{});
+    } else {
+#pragma unroll 1
+      for (int cz = 0; cz < 2; ++cz) zchan(cz);
     }
     if (li < Q) {
 #pragma unroll
