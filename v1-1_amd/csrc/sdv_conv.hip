@@ -312,7 +312,10 @@ __device__ __forceinline__ void wave_sync() {
 // ItemView<T>; base points at plane number `plane` (of 2m) of this pair.
 // With GUARD (blocks larger than NT) lanes at or beyond NT only take part in
 // the barriers.
-template <typename T, int P, int Q, bool GUARD, typename V>
+// FULL: the grid edge equals the circulant half-edge (n = m = R, e.g. 81^3),
+// so every row, column and position guard against n is known true at compile
+// time.
+template <typename T, int P, int Q, bool GUARD, typename V, bool FULL = false>
 __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, unsigned char* smem_raw) {
   constexpr int R = P * Q, M = 2 * R, pitch = R | 1;
   typedef YZCfg<T, P, Q> C;
@@ -328,7 +331,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   const int t = (!GUARD || threadIdx.x < NT) ? (int)threadIdx.x : (1 << 29);
   const int lane = t & 63, gi = lane / G, li = lane - gi * G;
   const int grp = gi < GPW ? (t >> 6) * GPW + gi : (1 << 28);  // lines grp, grp + NGB, ...
-  const int n = a.n;
+  const int n = FULL ? R : a.n;
   const int cx = plane / R, posx = plane - cx * R;
   const int kx = 2 * (posx / Q + P * (posx % Q)) + cx;
   const int kkx = min(kx, M - kx);
@@ -551,10 +554,11 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   }
 }
 
-template <typename T, int P, int Q>
+template <typename T, int P, int Q, bool FULL>
 __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_yz<T, P, Q, false>(a, a.X + ((size_t)blockIdx.y * 2 * (P * Q) + blockIdx.x) * a.n2, blockIdx.x, smem_raw);
+  body_yz<T, P, Q, false, KArgs<T>, FULL>(a, a.X + ((size_t)blockIdx.y * 2 * (P * Q) + blockIdx.x) * a.n2,
+                                          blockIdx.x, smem_raw);
 }
 
 // ---------------------------------------------------------------- x inverse
@@ -665,7 +669,9 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   const size_t shB = yz_lds_bytes(R, (int)sizeof(cplx<T>));
   SDV_REQUIRE(shB <= 160 * 1024 && shB >= (size_t)(a.lds_y + R * pitch) * sizeof(cplx<T>), SDV_ERR_UNSUPPORTED,
               "y/z plane does not fit in LDS");
-  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
+  const bool full = h.n == R;
+  const void* kyz = full ? (const void*)k_yz<T, P, Q, true> : (const void*)k_yz<T, P, Q, false>;
+  SDV_HIP_CHECK(hipFuncSetAttribute(kyz, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
   SDV_REQUIRE(shX <= 160 * 1024, SDV_ERR_UNSUPPORTED, "x lines do not fit in LDS");
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xfwd<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shX));
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xinv<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shX));
@@ -676,7 +682,10 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_yz<T, P, Q>), gB, dim3(YZCfg<T, P, Q>::NT), shB, st, a);
+  if (full)
+    hipLaunchKernelGGL((k_yz<T, P, Q, true>), gB, dim3(YZCfg<T, P, Q>::NT), shB, st, a);
+  else
+    hipLaunchKernelGGL((k_yz<T, P, Q, false>), gB, dim3(YZCfg<T, P, Q>::NT), shB, st, a);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
@@ -757,7 +766,10 @@ __device__ __noinline__ void call_xfwd(const ItemView<T>* v, int sub, unsigned c
 }
 template <typename T, int P, int Q>
 __device__ __noinline__ void call_yz(const ItemView<T>* v, int sub, unsigned char* smem) {
-  body_yz<T, P, Q, true>(*v, v->X + (size_t)sub * v->n2, sub, smem);
+  if (v->n == P * Q)  // the full-grid form, as the single-set and grouped paths choose it
+    body_yz<T, P, Q, true, ItemView<T>, true>(*v, v->X + (size_t)sub * v->n2, sub, smem);
+  else
+    body_yz<T, P, Q, true, ItemView<T>, false>(*v, v->X + (size_t)sub * v->n2, sub, smem);
 }
 template <typename T, int P, int Q>
 __device__ __noinline__ void call_xinv(const ItemView<T>* v, int sub, unsigned char* smem) {
@@ -841,7 +853,7 @@ __global__ void __launch_bounds__(kXThreads) k_xfwd_grp(const BSetDev<T>* __rest
   body_xfwd<T, P, Q>(batch_view(sets[it.set], it, X), wk.sub, smem_raw);
 }
 
-template <typename T, int P, int Q>
+template <typename T, int P, int Q, bool FULL>
 __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz_grp(const BSetDev<T>* __restrict__ sets,
                                                                const BItemDev* __restrict__ items,
                                                                const BWork* __restrict__ work, cplx<T>* X) {
@@ -849,7 +861,7 @@ __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz_grp(const BSetDev<T
   const BWork wk = work[blockIdx.x];
   const BItemDev it = items[wk.item];
   const ItemView<T> v = batch_view(sets[it.set], it, X);
-  body_yz<T, P, Q, false>(v, v.X + (size_t)wk.sub * v.n2, wk.sub, smem_raw);
+  body_yz<T, P, Q, false, ItemView<T>, FULL>(v, v.X + (size_t)wk.sub * v.n2, wk.sub, smem_raw);
 }
 
 template <typename T, int P, int Q>
@@ -877,15 +889,19 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
   long long xoff = 0;
   size_t shx = 0, shy = 0;
   int nty = 64;
+  // key = 2 m + (n == m): the grouped schedule launches one group per key, so
+  // sets whose grid edge equals the half-edge run the full-grid y/z kernel as
+  // they do in the single-set path (same per-pair arithmetic).
   struct Cost {
     long long c;
-    int m;
+    int key;
     BWork w;
   };
   std::vector<Cost> wx, wy;
   for (int si = 0; si < (int)sets.size(); ++si) {
     const BatchSetHost& s = sets[si];
     const int n = s.spec->g.n, m = s.spec->g.m;
+    const int key = 2 * m + (n == m ? 1 : 0);
     const int LA = lines_per_block(m);
     if (LA <= 0 || yz_threads(m, precision) <= 0 || m >= 128) {  // every context's m comes from choose_geom
       bp.items.clear();
@@ -908,14 +924,14 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
       bp.items.push_back(it);
       xoff += (long long)2 * m * n * n;
       const int nch = (n * n + LA - 1) / LA;
-      for (int c = 0; c < nch; ++c) wx.push_back(Cost{(long long)LA * m, m, BWork{ii, c}});
-      for (int pl = 0; pl < 2 * m; ++pl) wy.push_back(Cost{(long long)m * m * m, m, BWork{ii, pl}});
+      for (int c = 0; c < nch; ++c) wx.push_back(Cost{(long long)LA * m, key, BWork{ii, c}});
+      for (int pl = 0; pl < 2 * m; ++pl) wy.push_back(Cost{(long long)m * m * m, key, BWork{ii, pl}});
     }
   }
   // FUSED: largest cost first so the round-robin of the persistent grid stays
   // balanced; GROUPED: by size (largest first), each size contiguous.
   auto order = [mode](const Cost& a, const Cost& b) {
-    if (mode == BATCH_GROUPED && a.m != b.m) return a.m > b.m;
+    if (mode == BATCH_GROUPED && a.key != b.key) return a.key > b.key;
     if (a.c != b.c) return a.c > b.c;
     if (a.w.item != b.w.item) return a.w.item < b.w.item;
     return a.w.sub < b.w.sub;
@@ -923,7 +939,7 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
   // line chunks: chunk-major within a size, so the workgroups resident at
   // one time read the same orbital chunks (L2 reuse across pairs)
   auto order_x = [mode](const Cost& a, const Cost& b) {
-    if (mode == BATCH_GROUPED && a.m != b.m) return a.m > b.m;
+    if (mode == BATCH_GROUPED && a.key != b.key) return a.key > b.key;
     if (a.c != b.c) return a.c > b.c;
     if (a.w.sub != b.w.sub) return a.w.sub < b.w.sub;
     return a.w.item < b.w.item;
@@ -935,9 +951,11 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
   if (mode == BATCH_GROUPED) {
     for (size_t i = 0; i < wy.size();) {
       BatchGroup g;
-      g.m = wy[i].m;
+      const int key = wy[i].key;
+      g.m = key / 2;
+      g.full = (key & 1) != 0;
       g.wy0 = (int)i;
-      while (i < wy.size() && wy[i].m == g.m) ++i;
+      while (i < wy.size() && wy[i].key == key) ++i;
       g.nwy = (int)i - g.wy0;
       g.shx = lds_x_elems(g.m) * es;
       g.shy = lds_yz_elems(g.m, (int)es) * es;
@@ -945,10 +963,10 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
     }
     size_t gi = 0;
     for (size_t i = 0; i < wx.size(); ++gi) {
-      const int m = wx[i].m;
+      const int key = wx[i].key;
       const int b0 = (int)i;
-      while (i < wx.size() && wx[i].m == m) ++i;
-      if (gi >= bp.groups.size() || bp.groups[gi].m != m) {  // both lists hold the same sizes in the same order
+      while (i < wx.size() && wx[i].key == key) ++i;
+      if (gi >= bp.groups.size() || 2 * bp.groups[gi].m + (bp.groups[gi].full ? 1 : 0) != key) {  // same keys, same order
         bp.items.clear();
         bp.grid_x = bp.grid_y = 0;
         return bp;
@@ -998,8 +1016,8 @@ static int batch_launch_fused(const BatchPlan& bp, const BatchDev& d, hipStream_
 template <typename T, int P, int Q>
 static int launch_group(const BatchGroup& g, const BatchDev& d, hipStream_t st, Prof* prof) {
   SDV_REQUIRE(g.shy <= 160 * 1024, SDV_ERR_UNSUPPORTED, "batch plane does not fit in LDS");
-  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_yz_grp<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)g.shy));
+  const void* kyz = g.full ? (const void*)k_yz_grp<T, P, Q, true> : (const void*)k_yz_grp<T, P, Q, false>;
+  SDV_HIP_CHECK(hipFuncSetAttribute(kyz, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.shy));
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xfwd_grp<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)g.shx));
   SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xinv_grp<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1011,8 +1029,12 @@ static int launch_group(const BatchGroup& g, const BatchDev& d, hipStream_t st, 
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_yz_grp<T, P, Q>), dim3(g.nwy), dim3(YZCfg<T, P, Q>::NT), g.shy, st, sets, d.items,
-                     d.wy + g.wy0, X);
+  if (g.full)
+    hipLaunchKernelGGL((k_yz_grp<T, P, Q, true>), dim3(g.nwy), dim3(YZCfg<T, P, Q>::NT), g.shy, st, sets, d.items,
+                       d.wy + g.wy0, X);
+  else
+    hipLaunchKernelGGL((k_yz_grp<T, P, Q, false>), dim3(g.nwy), dim3(YZCfg<T, P, Q>::NT), g.shy, st, sets, d.items,
+                       d.wy + g.wy0, X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;

@@ -108,6 +108,7 @@ struct BatchGroup {
   int m = 0;
   int wx0 = 0, nwx = 0, wy0 = 0, nwy = 0;
   size_t shx = 0, shy = 0;
+  bool full = false;  // every set of the group has grid edge n = m (the y/z kernel's n guards drop)
 };
 // Schedules: FUSED = one persistent launch per stage over every set (block
 // shape and LDS of the largest size); GROUPED = one launch per stage per
