@@ -70,27 +70,40 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const z64* __restrict__ phi1,
     rab[q] = (r0 + rr < nrows) ? rows[r0 + rr] : int2{-1, -1};
     bcol[q] = (c0 + rr < ncols) ? tr + (size_t)cols[c0 + rr] * n3 : nullptr;
   }
+  // The next stage's orbitals and potentials are loaded into registers while
+  // the current stage's MFMAs run (register prefetch, one LDS buffer).
+  z64 fa[2], fw[2], fb[2];
+  auto fetch = [&](int xb) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int e = tid + 256 * q;
+      const int x = xb + e % kKC;
+      fa[q] = fw[q] = fb[q] = mk(0.0, 0.0);
+      if (x < x1) {
+        if (rab[q].x >= 0) {
+          fw[q] = phi2[(size_t)rab[q].y * n3 + x];
+          fa[q] = phi1[(size_t)rab[q].x * n3 + x];
+        }
+        if (bcol[q]) fb[q] = bcol[q][x];
+      }
+    }
+  };
+  fetch(x0);
   for (int xb = x0; xb < x1; xb += kKC) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int e = tid + 256 * q;
       const int xx = e % kKC, rr = e / kKC;
-      const int x = xb + xx;
-      z64 va = mk(0.0, 0.0), vb = mk(0.0, 0.0);
-      if (x < x1) {
-        if (rab[q].x >= 0) {  // rho = phi1(:,a) * CONJUGATE(phi2(:,b))   SD/coreproject.f90:1094
-          z64 w = phi2[(size_t)rab[q].y * n3 + x];
-          if (conj_b) w.im = -w.im;
-          va = phi1[(size_t)rab[q].x * n3 + x] * w;
-        }
-        if (bcol[q]) vb = bcol[q][x];
-      }
+      z64 w = fw[q];  // rho = phi1(:,a) * CONJUGATE(phi2(:,b))   SD/coreproject.f90:1094
+      if (conj_b) w.im = -w.im;
+      const z64 va = fa[q] * w, vb = fb[q];
       As[xx][2 * rr] = va.re;
       As[xx][2 * rr + 1] = va.im;
       Bs[xx][2 * rr] = vb.re;
       Bs[xx][2 * rr + 1] = vb.im;
     }
     __syncthreads();
+    if (xb + kKC < x1) fetch(xb + kKC);
 #pragma unroll
     for (int kk = 0; kk < kKC; kk += 4) {
       double a[2], b[2];
