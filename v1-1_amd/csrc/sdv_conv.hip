@@ -419,6 +419,17 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         acc[j1] = mk((T)0, (T)0);
       }
     }
+    // FP32 with P = Q: the nine T2 twiddles of this lane serve the forward and
+    // the inverse of both channels (T2[a P + b] is symmetric in a, b), so they
+    // are read from LDS once per column
+    constexpr bool T2C = sizeof(T) == 4 && P == Q;
+    cplx<T> t2c[T2C ? P : 1];
+    if (T2C && li < Q) {
+#pragma unroll
+      for (int k = 0; k < (T2C ? P : 1); ++k) t2c[k] = T2[li * P + k];
+    }
+    auto t2f = [&](int k1) -> cplx<T> { return T2C ? t2c[T2C ? k1 : 0] : T2[li * P + k1]; };
+    auto t2i = [&](int j2) -> cplx<T> { return T2C ? t2c[T2C ? j2 : 0] : T2[j2 * P + li]; };
     // one z channel: forward, spectral multiply, inverse, accumulated
     auto zchan = [&](auto czv) {
       const int cz = czv;
@@ -429,7 +440,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         for (int j1 = 0; j1 < P; ++j1) v[j1] = cz ? u[j1] * odd_tw(wb, j1) : u[j1];
         DFT<T, P, false>::run(v);
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * T2[li * P + k1];
+        for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * t2f(k1);
       }
       wave_sync();
       if (li < P) {  // step 2, spectral multiply, inverse step 1
@@ -444,7 +455,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         }
         DFT<T, Q, true>::run(w);
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) ycol[(li * Q + j2) * SR] = w[j2] * conj(T2[j2 * P + li]);
+        for (int j2 = 0; j2 < Q; ++j2) ycol[(li * Q + j2) * SR] = w[j2] * conj(t2i(j2));
       }
       wave_sync();
       if (li < Q) {  // inverse step 2; the odd channel carries w_M^-z

@@ -21,7 +21,11 @@
 
 namespace sdv {
 
-constexpr int kTM = 64, kTN = 64, kKC = 16;
+#ifndef SDV_MATEL_KKC
+#define SDV_MATEL_KKC 16
+#endif
+constexpr int kTM = 64, kTN = 64, kKC = SDV_MATEL_KKC;
+constexpr int kEPT = kKC * (kTM / 2) / 256;  // staged (grid point, pair) elements per thread and stage
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -62,20 +66,20 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const z64* __restrict__ phi1,
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
   // each thread stages two (grid point, pair) elements of A and of B per stage
-  int2 rab[2];
-  const z64* bcol[2];
+  int2 rab[kEPT];
+  const z64* bcol[kEPT];
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
+  for (int q = 0; q < kEPT; ++q) {
     const int rr = (tid + 256 * q) / kKC;
     rab[q] = (r0 + rr < nrows) ? rows[r0 + rr] : int2{-1, -1};
     bcol[q] = (c0 + rr < ncols) ? tr + (size_t)cols[c0 + rr] * n3 : nullptr;
   }
   // The next stage's orbitals and potentials are loaded into registers while
   // the current stage's MFMAs run (register prefetch, one LDS buffer).
-  z64 fa[2], fw[2], fb[2];
+  z64 fa[kEPT], fw[kEPT], fb[kEPT];
   auto fetch = [&](int xb) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < kEPT; ++q) {
       const int e = tid + 256 * q;
       const int x = xb + e % kKC;
       fa[q] = fw[q] = fb[q] = mk(0.0, 0.0);
@@ -91,7 +95,7 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(const z64* __restrict__ phi1,
   fetch(x0);
   for (int xb = x0; xb < x1; xb += kKC) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < kEPT; ++q) {
       const int e = tid + 256 * q;
       const int xx = e % kKC, rr = e / kKC;
       z64 w = fw[q];  // rho = phi1(:,a) * CONJUGATE(phi2(:,b))   SD/coreproject.f90:1094
