@@ -55,7 +55,10 @@ struct RowCol {
   }
 };
 
-constexpr int kXThreads = 512;
+#ifndef SDV_X_THREADS
+#define SDV_X_THREADS 512
+#endif
+constexpr int kXThreads = SDV_X_THREADS;
 
 template <typename T>
 struct KArgs {
@@ -131,10 +134,12 @@ struct XCfg {
 // A group could take LPG lines one after the other (smaller workgroups, two
 // per CU for FP32's 54 KB of LDS at m = 81); measured at 81^3 that is 1.4x
 // slower in FP32 than one line per group at 12 waves, so LPG = 1.
-// The y/z kernel stages its x-frequency plane's slice of the spectrum,
-// (m+1)^2 reals, in LDS after the plane when the workgroup's LDS still fits
-// (FP64 at m = 81: 109 KB plane and tables + 53 KB slice): the z pass then
-// reads it from LDS in the middle of its column FFTs instead of waiting on L2.
+// LDS after the plane.  FP32: the kernel stages its x-frequency plane's slice
+// of the spectrum, (m+1)^2 reals, when the workgroup's LDS still fits, so the
+// z pass reads it from LDS in the middle of its column FFTs.  FP64: the rows
+// of the parked cy = 0 output that fit in the rest of the 160 KB (42 of 81 at
+// m = 81, six whole wavefronts of rows) stay in LDS instead of HBM; measured
+// faster there than staging the spectrum (which was neutral in FP64).
 #ifndef SDV_YZ_KHAT_LDS
 #define SDV_YZ_KHAT_LDS 1
 #endif
@@ -145,10 +150,16 @@ constexpr size_t yz_base_bytes(int R, int es) {
   return (size_t)(3 * R + (R & 1) + R * (R | 1)) * es;  // tables (2R + R + pad) and plane, bytes
 }
 constexpr bool yz_khat_in_lds(int R, int es) {
-  return SDV_YZ_KHAT_LDS && yz_base_bytes(R, es) + (size_t)(R + 1) * (R + 1) * (es / 2) <= 160 * 1024;
+  return es == 8 && SDV_YZ_KHAT_LDS && yz_base_bytes(R, es) + (size_t)(R + 1) * (R + 1) * (es / 2) <= 160 * 1024;
+}
+// rows z < yz_park_rows park their cy = 0 output in LDS (FP64)
+constexpr int yz_park_rows(int R, int es) {
+  return es != 16 ? 0 : (int)((160 * 1024 - yz_base_bytes(R, es)) / ((size_t)R * es) < (size_t)R
+                                  ? (160 * 1024 - yz_base_bytes(R, es)) / ((size_t)R * es) : (size_t)R);
 }
 constexpr size_t yz_lds_bytes(int R, int es) {
-  return yz_base_bytes(R, es) + (yz_khat_in_lds(R, es) ? (size_t)(R + 1) * (R + 1) * (es / 2) : 0);
+  return yz_base_bytes(R, es) + (yz_khat_in_lds(R, es) ? (size_t)(R + 1) * (R + 1) * (es / 2) : 0) +
+         (size_t)yz_park_rows(R, es) * R * es;
 }
 
 template <typename T, int P, int Q>
@@ -288,13 +299,12 @@ __global__ void __launch_bounds__(kXThreads) k_xfwd(KArgs<T> a) {
 //             scratch of its own FFTs;
 //   rows    : y-inverse of row z, this channel's output part.
 // Five workgroup barriers per plane.  The cy = 0 output part stays in
-// registers (FP32) or is parked in the plane's own HBM slots, which the same
-// lanes have just read for the cy = 1 input (FP64, whose registers are
-// needed for the column FFTs); the parked values are loaded before the last
-// y-inverse, so their latency overlaps it.  Measured alternatives at 81^3
-// (FP64): parking the rows that fit in the LDS left over by the plane adds
-// register spills and runs slower; row pitches other than R | 1 are no
-// faster (even pitches are 18 % slower).
+// registers (FP32) or is parked (FP64, whose registers are needed for the
+// column FFTs): rows z < yz_park_rows in LDS after the plane, each lane in
+// slots of its own, the other rows in the plane's own HBM slots, which the
+// same lanes have just read for the cy = 1 input; the parked values are loaded
+// before the last y-inverse, so their latency overlaps it.  Row pitches other
+// than R | 1 were measured no faster (even pitches 18 % slower).
 // The LDS offsets are runtime values, so the compiler cannot prove the
 // twiddle tables distinct from the plane and does not pin hoisted twiddles
 // in VGPRs across phases.
@@ -530,7 +540,14 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       yfwd(1, g, x);
     } else {
       yfwd(1, g, x);  // consumes x before its HBM slots take the parked cy = 0 output
-      if (g < n && li < Q) {
+      constexpr int PR = yz_park_rows(R, (int)sizeof(cplx<T>));
+      cplx<T>* park = Y + R * pitch;
+      if (g < PR) {
+        if (li < Q) {
+#pragma unroll
+          for (int j1 = 0; j1 < P; ++j1) park[(g * Q + li) * P + j1] = o[j1];
+        }
+      } else if (g < n && li < Q) {
 #pragma unroll
         for (int j1 = 0; j1 < P; ++j1) {
           const int y = j1 * Q + li;
@@ -547,7 +564,13 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   for (int r = 0; r < LPG; ++r) {
     const int g = grp + r * NGB;
     cplx<T> o[P], pk[P];
-    if (!KEEP && g < n && li < Q) {  // the parked cy = 0 part, in flight during the y-inverse
+    constexpr int PR = yz_park_rows(R, (int)sizeof(cplx<T>));
+    if (!KEEP && g < PR) {
+      if (li < Q) {
+#pragma unroll
+        for (int j1 = 0; j1 < P; ++j1) pk[j1] = (Y + R * pitch)[(g * Q + li) * P + j1];
+      }
+    } else if (!KEEP && g < n && li < Q) {  // the parked cy = 0 part, in flight during the y-inverse
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
         const int y = j1 * Q + li;

@@ -21,7 +21,11 @@ SOURCES = ["sdv_abi.hip", "sdv_conv.hip", "sdv_spectrum.hip", "sdv_matel.hip", "
            "sdv_meanfield.hip", "sdv_server.hip", "sdv_onebody.hip"]
 
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 instructions.
-FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result",
+# -ffp-contract=on: multiply-adds fuse only within one source expression, so
+# the rounding of every kernel follows its source, not how the compiler laid
+# out the surrounding control flow; the batched, serving and single-set forms
+# share device functions and stay bitwise equal.
+FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-ffp-contract=on", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-value", "-Wno-unused-result",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-I" + os.path.join(REPO, "include"), "-I" + CSRC]
 
