@@ -177,3 +177,40 @@ def test_krylov_step_matches_oracle(orc, torch_dev, sysm):
     assert relerr(H, Hr) < 1e-10
     g = got.reshape(N, n3).cpu().numpy().T.reshape(-1, order="F")
     assert relerr(g, ref) < 1e-10
+
+
+def test_phiv_orbital_step_matches_oracle(orc, torch_dev, sysm):
+    """One interval of the reference's orbital step phi1 = phi0 + t phi(tJ) f(phi0)
+    (S/expo_driver.f90:628-650: DGPHIV with v = 0, u = the derivative) with
+    the device operator as J, against the same adaptive driver on the oracle
+    operator; the step sizes it chooses are reported by both."""
+    torch = torch_dev
+    from sincdvr import krylov
+    m = sysm
+    n3, N = m["n"] ** 3, m["N"]
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).cuda()      # noqa: E731
+    dm, idm, rp, pot = T(m["denmat"]), T(m["invden"]), T(m["rp"]), torch.from_numpy(m["pot"]).cuda()
+    proj = T(m["phi"])
+
+    def apply_dev(v):
+        return m["s"].actreduced00(v.reshape(N, n3), dm, rp, pot=pot, invdenmat=idm, timefac=-1j,
+                                   projspfs=proj).reshape(-1)
+
+    def apply_ref(v):
+        x = np.asfortranarray(v.numpy().reshape(n3, N, order="F"))
+        y = orc.actreduced00(m["n"], m["ke"], x, m["denmat"], m["rp"], pot=m["pot"], invdenmat=m["invden"],
+                             timefac=-1j, projspfs=m["phi"])
+        return torch.from_numpy(np.ascontiguousarray(y.reshape(-1, order="F")))
+
+    phi0 = np.asfortranarray(m["phi"] * (1 + 0.1j))
+    d0 = T(phi0).reshape(-1)
+    r0 = torch.from_numpy(np.ascontiguousarray(phi0.reshape(-1, order="F")))
+    t, tol, order, guess = 0.05, 1e-9, 8, 0.01
+    say("phiv orbital step t=%g tol=%g m=%d" % (t, tol, order))
+    wd, sd = krylov.phiv(apply_dev, t, apply_dev(d0), torch.zeros_like(d0), tol=tol, m=order, guess_step=guess)
+    wr, sr = krylov.phiv(apply_ref, t, apply_ref(r0), torch.zeros_like(r0), tol=tol, m=order, guess_step=guess)
+    say("device steps %d (rejected %d), oracle steps %d (rejected %d)"
+        % (sd["nstep"], sd["nreject"], sr["nstep"], sr["nreject"]))
+    got = wd.reshape(N, n3).cpu().numpy().T.reshape(-1, order="F")
+    assert abs(sd["t_now"] - t) < 1e-14 and abs(sr["t_now"] - t) < 1e-14
+    assert relerr(got, wr.numpy()) < 1e-7
