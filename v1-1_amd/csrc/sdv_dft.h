@@ -89,15 +89,12 @@ struct DFT<T, 3, INV> {
     cplx<T> t1 = v[1] + v[2];
     cplx<T> t2 = mk(v[0].re - (T)0.5 * t1.re, v[0].im - (T)0.5 * t1.im);
     cplx<T> d = v[1] - v[2];
-    cplx<T> sd = mk(s * d.im, -s * d.re);  // -i s d
     v[0] = v[0] + t1;
-    if (!INV) {
-      v[1] = t2 + sd;
-      v[2] = t2 - sd;
-    } else {
-      v[1] = t2 - sd;
-      v[2] = t2 + sd;
-    }
+    // t2 -+ i s d, each component one expression (one fused multiply-add)
+    const cplx<T> p = mk(t2.re + s * d.im, t2.im - s * d.re);
+    const cplx<T> q = mk(t2.re - s * d.im, t2.im + s * d.re);
+    v[1] = INV ? q : p;
+    v[2] = INV ? p : q;
   }
 };
 

@@ -15,6 +15,10 @@ summarize: reads rocprofv3 output (kernel-trace stats CSV, counter CSVs of a
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- python3 v1-1_amd/sincdvr/profiling.py run
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- python3 v1-1_amd/sincdvr/profiling.py run
     python3 v1-1_amd/sincdvr/profiling.py summarize OUT profiles
+
+sq: the SQ counter table (VALU and LDS instructions, waits, LDS bank
+     conflicts, VALU busy) of the convolution kernels from one counter pass
+     per precision (summarize_sq gives the commands).
 """
 import argparse
 import csv
@@ -149,6 +153,60 @@ def summarize(out_dir, prof_dir, tag="pairpot_81_n16"):
     return lines
 
 
+SQ_COUNTERS = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
+               "SQ_LDS_IDX_ACTIVE", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES")
+SIMDS = 1024                  # 256 CUs x 4 SIMDs
+YZ_WAVES_PER_SIMD = 3         # 9x9 y/z workgroup: 768 threads, one workgroup per CU (LDS)
+
+
+def summarize_sq(out_dir, prof_dir):
+    """Table of SQ counters per full dispatch from the passes OUT/sq0 (FP64)
+    and OUT/sq1 (mixed).  SQ_WAVE_CYCLES counts quad cycles; one VALU
+    instruction of a wavefront takes one quad cycle on its SIMD (FP64 and
+    unpacked FP32 alike), so VALU busy = VALU instructions per SIMD over the
+    SIMD quad cycles of the dispatch (its wavefront cycles over SIMDS x
+    resident waves per SIMD).  Each pass lists the kernels of its own element
+    type (the mixed pass also runs FP64 convolutions to build its carrier)."""
+    lines = [
+        "# rocprofv3 SQ counters: k_yz / k_xfwd / k_xinv at 81^3, 16 orbitals (%d convolutions per dispatch)"
+        % CONV_PER_LAUNCH, "",
+        "Commands (repository root, on the GPU; one pass per precision P = 0 FP64, 1 mixed):", "",
+        "    rocprofv3 --pmc %s --output-format csv -d OUT/sqP -o run -- python3 v1-1_amd/sincdvr/profiling.py "
+        "run --precision P --steps 1" % " ".join(SQ_COUNTERS),
+        "    python3 v1-1_amd/sincdvr/profiling.py sq OUT profiles", "",
+        "Per dispatch of the full %d-convolution grid (the largest grid of each kernel), summed over the" % CONV_PER_LAUNCH,
+        "device.  Waits are fractions of wavefront cycles; bank conflicts are per LDS-active cycle.",
+        "VALU busy = VALU instructions (one quad cycle each) over the SIMDs' quad cycles, with the",
+        "dispatch's wavefront cycles spread over %d SIMDs at the resident waves per SIMD (%d for k_yz)."
+        % (SIMDS, YZ_WAVES_PER_SIMD), "",
+        "| kernel | VALU insts | LDS insts | wait any / wave cycles | wait issue / wave cycles | "
+        "LDS bank conflict / LDS active |",
+        "|---|---:|---:|---:|---:|---:|"]
+    for p, label in ((0, "FP64"), (1, "FP32")):
+        rows = _rows(os.path.join(out_dir, f"sq{p}", "**", "*counter_collection.csv"))
+        c = {name: counter_per_dispatch(rows, name) for name in SQ_COUNTERS}
+        busy = None
+        elem = "<double," if p == 0 else "<float,"
+        for k in sorted(c["SQ_INSTS_VALU"]):
+            if elem not in k:
+                continue
+            wc = c["SQ_WAVE_CYCLES"].get(k, 0.0)
+            lds_act = c["SQ_LDS_IDX_ACTIVE"].get(k, 0.0)
+            lines.append("| %s | %.3e | %.3e | %.2f | %.2f | %.2f |" % (
+                k, c["SQ_INSTS_VALU"][k], c["SQ_INSTS_LDS"].get(k, 0.0),
+                c["SQ_WAIT_ANY"].get(k, 0.0) / wc if wc else 0.0,
+                c["SQ_WAIT_INST_ANY"].get(k, 0.0) / wc if wc else 0.0,
+                c["SQ_LDS_BANK_CONFLICT"].get(k, 0.0) / lds_act if lds_act else 0.0))
+            if k.startswith("sdv::k_yz<") and wc:
+                busy = (c["SQ_INSTS_VALU"][k] / SIMDS) / (wc / (SIMDS * YZ_WAVES_PER_SIMD))
+        if busy is not None:
+            lines.append("|  (VALU busy %s) | %.2f | | | | |" % (label, busy))
+    os.makedirs(prof_dir, exist_ok=True)
+    with open(os.path.join(prof_dir, "k_yz_81_n16_sq_counters.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -159,9 +217,14 @@ def main():
     sm.add_argument("out_dir")
     sm.add_argument("prof_dir")
     sm.add_argument("--tag", default="pairpot_81_n16")
+    sq = sub.add_parser("sq")
+    sq.add_argument("out_dir")
+    sq.add_argument("prof_dir")
     a = ap.parse_args()
     if a.cmd == "run":
         run(a.precision, a.steps)
+    elif a.cmd == "sq":
+        print("\n".join(summarize_sq(a.out_dir, a.prof_dir)))
     else:
         print("\n".join(summarize(a.out_dir, a.prof_dir, a.tag)))
 
