@@ -277,7 +277,38 @@ def test_twoe_matel_16_orbitals_small_grid(orc, torch_dev, lowhigh):
     assert relerr(tm, otm) < TOL64
 
 
-@pytest.mark.parametrize("realkind4", [False, True])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("cnorm", [False, True])
+def test_single_orbital_and_range_edges(orc, torch_dev, cnorm, precision):
+    """One orbital: a single diagonal pair, which for hermnorm has no partner
+    to share a convolution with and runs alone.  An empty spf range (high <
+    low: an empty loop at SD/coreproject.f90:1042) computes nothing and leaves
+    the output as it was; a range outside 1..numspf and a grid edge beyond
+    the largest circulant half-edge (81) are errors."""
+    from sincdvr import SdvError, SincDVR, inputs
+    n, dx = 21, 0.4
+    phi = inputs.gaussian_orbitals(n, dx, 1, 3, seed=91, cnorm=cnorm)
+    say("single orbital n=21 cnorm=%d precision=%d" % (cnorm, precision))
+    s = SincDVR(n, dx, 1, cnorm=cnorm, precision=precision)
+    tm, tr = s.call_twoe_matel00(1, 1, phi, phi)
+    otm, otr = orc.twoe_matel(n, orc.coulomb_kernel(n, dx), phi, phi, cnorm=cnorm)
+    tol = TOL64 if precision == 0 else 2e-6
+    assert relerr(tr, otr) < tol
+    assert relerr(tm, otm) < tol
+    if not cnorm:  # |phi|^2 convolved with a real kernel: real up to rounding
+        assert np.abs(tr.imag).max() <= tol * np.abs(tr).max()
+    p = torch_dev.from_numpy(np.ascontiguousarray(phi.T)).cuda()
+    keep = torch_dev.full((1, 1, n ** 3), 7.0 + 3.0j, dtype=torch_dev.complex128, device="cuda")
+    s.twoereduced(2, 1, p, out=keep)
+    torch_dev.cuda.synchronize()
+    assert bool((keep == 7.0 + 3.0j).all())
+    for low, high in ((0, 1), (1, 2)):
+        with pytest.raises(SdvError) as e:
+            s.twoereduced(low, high, p, out=keep)
+        assert e.value.code == -1
+    with pytest.raises(SdvError) as e:
+        SincDVR(82, dx, 1, precision=precision)
+    assert e.value.code == -3
 def test_nuclear_potential(orc, torch_dev, realkind4):
     """Two nuclei of charge 1 at +-0.6 bohr on z (centershift +-4) on the
     41^3, 0.3-bohr grid, and one charge-2 nucleus at the origin on 21^3."""

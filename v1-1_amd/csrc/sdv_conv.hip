@@ -133,7 +133,10 @@ struct XCfg {
 // per wave), so the exchanges inside a line FFT need no workgroup barrier.
 // A group could take LPG lines one after the other (smaller workgroups, two
 // per CU for FP32's 54 KB of LDS at m = 81); measured at 81^3 that is 1.4x
-// slower in FP32 than one line per group at 12 waves, so LPG = 1.
+// slower in FP32 than one line per group at 12 waves, and 1.2x slower in FP64
+// (6 waves, no register spill, against 32 bytes per lane of spill at 12), so
+// LPG = 1.  Static wave priorities that differ between the three waves of a
+// SIMD (to stagger their LDS and VALU phases) change nothing measurable.
 // LDS after the plane.  FP32: the kernel stages its x-frequency plane's slice
 // of the spectrum, (m+1)^2 reals, when the workgroup's LDS still fits, so the
 // z pass reads it from LDS in the middle of its column FFTs.  FP64: the rows
