@@ -309,6 +309,9 @@ def test_single_orbital_and_range_edges(orc, torch_dev, cnorm, precision):
     with pytest.raises(SdvError) as e:
         SincDVR(82, dx, 1, precision=precision)
     assert e.value.code == -3
+
+
+@pytest.mark.parametrize("realkind4", [False, True])
 def test_nuclear_potential(orc, torch_dev, realkind4):
     """Two nuclei of charge 1 at +-0.6 bohr on z (centershift +-4) on the
     41^3, 0.3-bohr grid, and one charge-2 nucleus at the origin on 21^3."""

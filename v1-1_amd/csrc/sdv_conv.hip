@@ -63,6 +63,7 @@ constexpr int kXThreads = SDV_X_THREADS;
 template <typename T>
 struct KArgs {
   int n, n2, n3, npairs;
+  int ld;              // plane stride of X (plane_stride(n) >= n2)
   const z64* phi1;
   const z64* phi2;
   int density_mode, conj_b, mirror_conj;
@@ -81,6 +82,7 @@ struct KArgs {
 template <typename T>
 struct ItemView {
   int n, n2, n3;
+  int ld;              // plane stride of X (plane_stride(n) >= n2)
   const z64* pa;       // phi1 column a (orbital mode) or density column (density mode)
   const z64* pb;       // phi2 column b
   int density_mode, conj_b, mirror_conj;
@@ -102,12 +104,13 @@ __device__ __forceinline__ ItemView<T> make_view(const KArgs<T>& a, int pidx, in
   v.n = a.n;
   v.n2 = a.n2;
   v.n3 = a.n3;
+  v.ld = a.ld;
   v.pa = a.phi1 + (size_t)it.a * a.n3;
   v.pb = a.phi2 + (size_t)it.b * a.n3;
   v.density_mode = a.density_mode;
   v.conj_b = a.conj_b;
   v.mirror_conj = a.mirror_conj;
-  v.X = a.X + (size_t)pidx * 2 * R * a.n2;
+  v.X = a.X + (size_t)pidx * 2 * R * a.ld;
   v.out = a.out + (size_t)it.col * a.n3;
   v.outm = it.mcol >= 0 ? a.out + (size_t)it.mcol * a.n3 : nullptr;
   v.tw = a.tw;
@@ -187,9 +190,9 @@ __device__ __forceinline__ void xfwd_step2(const ItemView<T>& a, const cplx<T>* 
 #pragma unroll
     for (int j2 = 0; j2 < Q; ++j2) u[j2] = buf[line * pitch + k1 * Q + j2];
     DFT<T, Q, false>::run(u);
-    cplx<T>* dst = a.X + ((size_t)c * R + (size_t)k1 * Q) * a.n2 + l0 + line;
+    cplx<T>* dst = a.X + ((size_t)c * R + (size_t)k1 * Q) * a.ld + l0 + line;
 #pragma unroll
-    for (int k2 = 0; k2 < Q; ++k2) dst[(size_t)k2 * a.n2] = u[k2];
+    for (int k2 = 0; k2 < Q; ++k2) dst[(size_t)k2 * a.ld] = u[k2];
   }
 }
 
@@ -594,7 +597,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 template <typename T, int P, int Q, bool FULL>
 __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_yz<T, P, Q, false, KArgs<T>, FULL>(a, a.X + ((size_t)blockIdx.y * 2 * (P * Q) + blockIdx.x) * a.n2,
+  body_yz<T, P, Q, false, KArgs<T>, FULL>(a, a.X + ((size_t)blockIdx.y * 2 * (P * Q) + blockIdx.x) * a.ld,
                                           blockIdx.x, smem_raw);
 }
 
@@ -616,9 +619,9 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
 #pragma unroll
   for (int j = 0; j < P; ++j) acc[j] = mk((T)0, (T)0);
   for (int c = 0; c < 2; ++c) {
-    const cplx<T>* src = a.X + (size_t)c * R * a.n2 + l0;
+    const cplx<T>* src = a.X + (size_t)c * R * a.ld + l0;
     RowCol rc(tid, XCfg<P, Q>::XT, nl);
-    for (int e = tid; e < R * nl; e += XCfg<P, Q>::XT, rc.next()) buf[rc.col * pitch + rc.row] = src[(size_t)rc.row * a.n2 + rc.col];
+    for (int e = tid; e < R * nl; e += XCfg<P, Q>::XT, rc.next()) buf[rc.col * pitch + rc.row] = src[(size_t)rc.row * a.ld + rc.col];
     __syncthreads();
     if (jj < P && line < nl) {  // inverse step 1: task (line, k1)
       cplx<T> u[Q];
@@ -686,6 +689,7 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   a.n = h.n;
   a.n2 = h.n * h.n;
   a.n3 = h.n * h.n * h.n;
+  a.ld = plane_stride(h.n);
   a.npairs = h.npairs;
   a.phi1 = (const z64*)h.phi1;
   a.phi2 = (const z64*)h.phi2;
@@ -754,6 +758,7 @@ __device__ __forceinline__ ItemView<T> batch_view(const BSetDev<T>& s, const BIt
   v.n = s.n;
   v.n2 = s.n * s.n;
   v.n3 = s.n * s.n * s.n;
+  v.ld = plane_stride(s.n);
   v.pa = s.phi + (size_t)it.a * v.n3;
   v.pb = s.phi + (size_t)it.b * v.n3;
   v.density_mode = 0;
@@ -804,9 +809,9 @@ __device__ __noinline__ void call_xfwd(const ItemView<T>* v, int sub, unsigned c
 template <typename T, int P, int Q>
 __device__ __noinline__ void call_yz(const ItemView<T>* v, int sub, unsigned char* smem) {
   if (v->n == P * Q)  // the full-grid form, as the single-set and grouped paths choose it
-    body_yz<T, P, Q, true, ItemView<T>, true>(*v, v->X + (size_t)sub * v->n2, sub, smem);
+    body_yz<T, P, Q, true, ItemView<T>, true>(*v, v->X + (size_t)sub * v->ld, sub, smem);
   else
-    body_yz<T, P, Q, true, ItemView<T>, false>(*v, v->X + (size_t)sub * v->n2, sub, smem);
+    body_yz<T, P, Q, true, ItemView<T>, false>(*v, v->X + (size_t)sub * v->ld, sub, smem);
 }
 template <typename T, int P, int Q>
 __device__ __noinline__ void call_xinv(const ItemView<T>* v, int sub, unsigned char* smem) {
@@ -898,7 +903,7 @@ __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz_grp(const BSetDev<T
   const BWork wk = work[blockIdx.x];
   const BItemDev it = items[wk.item];
   const ItemView<T> v = batch_view(sets[it.set], it, X);
-  body_yz<T, P, Q, false, ItemView<T>, FULL>(v, v.X + (size_t)wk.sub * v.n2, wk.sub, smem_raw);
+  body_yz<T, P, Q, false, ItemView<T>, FULL>(v, v.X + (size_t)wk.sub * v.ld, wk.sub, smem_raw);
 }
 
 template <typename T, int P, int Q>
@@ -959,7 +964,7 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
       it.xoff = xoff;
       const int ii = (int)bp.items.size();
       bp.items.push_back(it);
-      xoff += (long long)2 * m * n * n;
+      xoff += (long long)2 * m * plane_stride(n);
       const int nch = (n * n + LA - 1) / LA;
       for (int c = 0; c < nch; ++c) wx.push_back(Cost{(long long)LA * m, key, BWork{ii, c}});
       for (int pl = 0; pl < 2 * m; ++pl) wy.push_back(Cost{(long long)m * m * m, key, BWork{ii, pl}});
@@ -1180,7 +1185,7 @@ int circulant_sizes(int* out, int cap) {
 
 size_t conv_work_bytes(const Geom& g, int npairs, int precision) {
   const size_t es = precision == 0 ? sizeof(z64) : sizeof(c32);
-  return (size_t)npairs * 2 * g.m * (size_t)g.n * g.n * es;
+  return (size_t)npairs * 2 * g.m * (size_t)plane_stride(g.n) * es;
 }
 
 template <typename T>

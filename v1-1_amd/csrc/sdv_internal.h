@@ -77,9 +77,15 @@ struct ConvArgsHost {
   int conj_b;         // 1: conjugate the second factor (hermnorm)
   int mirror_conj;    // 1: mirror columns receive conj(V)
   const PairItem* pairs_dev;
-  void* work;         // intermediate, npairs * 2m * n^2 complex<T>
+  void* work;         // intermediate, npairs * 2m planes of plane_stride(n) complex<T>
   void* out;          // z64 (n^3, ncols)
 };
+
+// Elements between consecutive x-frequency planes of the convolution work
+// buffer: the n^2 lines of a plane rounded up to 16, so every plane starts on
+// a 256-byte boundary and the x kernels' runs of consecutive lines (56 FP64
+// lines = 7 cache lines at m = 81) do not straddle an extra cache line.
+__host__ __device__ inline int plane_stride(int n) { return (n * n + 15) & ~15; }
 
 // Convolution pipeline: x-forward (pair density fused), y/z plane kernel with
 // the spectral multiply, x-inverse with output scatter.  precision 0 = FP64,

@@ -131,7 +131,7 @@ struct sdv_server {
 };
 
 static size_t pair_work_bytes(const Geom& g, int precision) {
-  return (size_t)2 * g.m * g.n * g.n * (precision == 0 ? sizeof(z64) : sizeof(c32));
+  return (size_t)2 * g.m * plane_stride(g.n) * (precision == 0 ? sizeof(z64) : sizeof(c32));
 }
 
 static int grow_slot_tables(Slot& s, size_t bytes) {
