@@ -124,6 +124,18 @@ __device__ __forceinline__ ItemView<T> make_view(const KArgs<T>& a, int pidx, in
   return v;
 }
 
+// Workgroups of one dispatch go to the 8 XCDs round robin in the order of
+// their linear index, and each XCD has its own L2.  xcd_span renumbers the
+// linear index L of a grid of `total` workgroups so that the workgroups one
+// XCD receives form one contiguous range of the logical order (XCD x takes
+// the x-th of 8 near-equal ranges).  It is a bijection of [0, total) whatever
+// the hardware order, so only cache reuse depends on it.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_span(unsigned L, unsigned total) {
+  const unsigned x = L % kXcds, j = L / kXcds, q = total / kXcds, r = total % kXcds;
+  return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+}
+
 template <int P, int Q>
 struct XCfg {
   static constexpr int MX = P > Q ? P : Q;
@@ -283,10 +295,15 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
 template <typename T, int P, int Q>
 __global__ void __launch_bounds__(kXThreads) k_xfwd(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  // pairs run fastest across the grid: the workgroups resident at one time
-  // work on the same line chunk of many pairs, so each orbital chunk is
-  // fetched from HBM once and served to its other pairs from L2
-  body_xfwd<T, P, Q>(make_view(a, blockIdx.x, P * Q), blockIdx.y, smem_raw);
+  // pairs run fastest in the logical order, so the workgroups resident at
+  // one time work on the same line chunk of many pairs, and each XCD takes
+  // one range of line chunks, so each orbital chunk is fetched into one
+  // XCD's L2 and served to all its pairs there (pair-fastest over the raw
+  // index spreads every chunk's pairs over all 8 XCDs and fetches the
+  // orbitals 8 times: 1.08 GB per launch at 81^3 with 16 orbitals;
+  // 0.60 -> 0.57 ms FP64, FP32 unchanged)
+  const unsigned L = xcd_span(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  body_xfwd<T, P, Q>(make_view(a, L % gridDim.x, P * Q), L / gridDim.x, smem_raw);
 }
 
 // ------------------------------------------------------------- y/z planes
