@@ -190,7 +190,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="headline forms only")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is described in DESIGN.md",
+        print("bench.py: this library runs the pair potentials on one GPU; --gpus must be 1 "
+              "(pairs shard across GPUs by unique-pair ranges, which this benchmark does not run)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -199,7 +200,7 @@ def main():
 
     from sincdvr import SincDVR, inputs
 
-    cfg = inputs.CONFIGS["c3_81"]
+    cfg = inputs.CONFIGS["grid81_16orb"]
     n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
     npairs = N * (N + 1) // 2
     print(f"[bench] generating {N} orbitals on {n}^3", file=sys.stderr, flush=True)

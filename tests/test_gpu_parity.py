@@ -28,22 +28,22 @@ def torch_dev():
 
 
 @pytest.fixture(scope="module")
-def c0(orc):
+def w21(orc):
     from sincdvr import inputs
-    cfg = inputs.CONFIGS["c0_21"]
+    cfg = inputs.CONFIGS["grid21_4orb"]
     kern = orc.coulomb_kernel(cfg["n"], cfg["spacing"])
     phi_h = inputs.gaussian_orbitals(cfg["n"], cfg["spacing"], cfg["norb"], cfg["ngauss"], cfg["seed"], cnorm=False)
     phi_c = inputs.gaussian_orbitals(cfg["n"], cfg["spacing"], cfg["norb"], cfg["ngauss"], cfg["seed"], cnorm=True)
     return dict(cfg=cfg, kern=kern, phi_h=phi_h, phi_c=phi_c)
 
 
-def test_kernel_and_stencil_parity(orc, torch_dev, c0):
+def test_kernel_and_stencil_parity(orc, torch_dev, w21):
     from sincdvr import SincDVR
-    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    n, dx = w21["cfg"]["n"], w21["cfg"]["spacing"]
     say("kernel octant n=%d" % n)
     s = SincDVR(n, dx, 4)
     ko = s.kernel_octant()
-    ref = c0["kern"][n - 1:, n - 1:, n - 1:]
+    ref = w21["kern"][n - 1:, n - 1:, n - 1:]
     assert relerr(ko, ref) < 1e-13
     ke, _, _ = orc.sinedvr(n, dx)
     assert relerr(s.ke_stencil(), ke) < 1e-15
@@ -51,14 +51,14 @@ def test_kernel_and_stencil_parity(orc, torch_dev, c0):
 
 
 @pytest.mark.parametrize("cnorm", [False, True])
-def test_twoe_matel_c0_fp64(orc, torch_dev, c0, cnorm):
+def test_twoe_matel_grid21_fp64(orc, torch_dev, w21, cnorm):
     from sincdvr import SincDVR
-    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
-    phi = c0["phi_c"] if cnorm else c0["phi_h"]
+    n, dx = w21["cfg"]["n"], w21["cfg"]["spacing"]
+    phi = w21["phi_c"] if cnorm else w21["phi_h"]
     say("call_twoe_matel00 n=%d N=4 cnorm=%s host pointers" % (n, cnorm))
     s = SincDVR(n, dx, 4, cnorm=cnorm)
     tm, tr = s.call_twoe_matel00(1, 4, phi, phi)
-    otm, otr = orc.twoe_matel(n, c0["kern"], phi, phi, cnorm=cnorm)     # reference layout M = 2n
+    otm, otr = orc.twoe_matel(n, w21["kern"], phi, phi, cnorm=cnorm)     # reference layout M = 2n
     assert relerr(tr, otr) < TOL64
     assert relerr(tm, otm) < TOL64
     # device pointers give the same numbers
@@ -73,39 +73,39 @@ def test_twoe_matel_c0_fp64(orc, torch_dev, c0, cnorm):
 
 @pytest.mark.parametrize("lowhigh", [(2, 3), (4, 4), (1, 1)])
 @pytest.mark.parametrize("same", [True, False])
-def test_twoe_matel_ranges_and_general_pairs(orc, torch_dev, c0, lowhigh, same):
+def test_twoe_matel_ranges_and_general_pairs(orc, torch_dev, w21, lowhigh, same):
     from sincdvr import SincDVR
-    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
-    phi1 = c0["phi_h"]
+    n, dx = w21["cfg"]["n"], w21["cfg"]["spacing"]
+    phi1 = w21["phi_h"]
     phi2 = phi1 if same else np.asfortranarray(phi1[:, ::-1] * (0.5 + 0.25j))
     low, high = lowhigh
     say("call_twoe_matel00 low=%d high=%d same=%s" % (low, high, same))
     s = SincDVR(n, dx, 4)
     tm, tr = s.call_twoe_matel00(low, high, phi1, phi2)
-    otm, otr = orc.twoe_matel(n, c0["kern"], phi1, phi2, low=low, high=high)
+    otm, otr = orc.twoe_matel(n, w21["kern"], phi1, phi2, low=low, high=high)
     assert tm.shape == otm.shape and tr.shape == otr.shape
     assert relerr(tr, otr) < TOL64
     assert relerr(tm, otm) < TOL64
 
 
-def test_flags_notwo_and_debug(orc, torch_dev, c0):
+def test_flags_notwo_and_debug(orc, torch_dev, w21):
     from sincdvr import SincDVR
-    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    n, dx = w21["cfg"]["n"], w21["cfg"]["spacing"]
     say("notwoflag / debugflag=4242")
-    tm, tr = SincDVR(n, dx, 4, notwoflag=1).call_twoe_matel00(1, 4, c0["phi_h"], c0["phi_h"])
+    tm, tr = SincDVR(n, dx, 4, notwoflag=1).call_twoe_matel00(1, 4, w21["phi_h"], w21["phi_h"])
     assert not tm.any() and not tr.any()
-    den = np.asfortranarray(c0["phi_h"] * 1.0)
+    den = np.asfortranarray(w21["phi_h"] * 1.0)
     assert not SincDVR(n, dx, 4, debugflag=4242).op_tinv(den).any()
 
 
-def test_op_tinv_size_error(torch_dev, c0):
+def test_op_tinv_size_error(torch_dev, w21):
     from sincdvr import SdvError, SincDVR
-    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    n, dx = w21["cfg"]["n"], w21["cfg"]["spacing"]
     s = SincDVR(n, dx, 4)
     with pytest.raises(SdvError):
         s.op_tinv(np.zeros((n ** 3, 3), dtype=complex, order="F"), circsize=2)
     with pytest.raises(SdvError):
-        s.call_twoe_matel00(0, 4, c0["phi_h"], c0["phi_h"])
+        s.call_twoe_matel00(0, 4, w21["phi_h"], w21["phi_h"])
 
 
 @pytest.mark.parametrize("n", [5, 9, 12, 16, 21, 25, 27, 32, 36, 45, 48, 54, 63, 64, 72, 81])
@@ -152,9 +152,9 @@ def test_mult_ke(orc, torch_dev, ecs, n):
     assert e32 < 2e-6
 
 
-def test_mean_field(orc, torch_dev, c0):
+def test_mean_field(orc, torch_dev, w21):
     from sincdvr import SincDVR
-    n, dx, N = c0["cfg"]["n"], c0["cfg"]["spacing"], 4
+    n, dx, N = w21["cfg"]["n"], w21["cfg"]["spacing"], 4
     rng = np.random.default_rng(9)
     tr = np.asfortranarray(rng.standard_normal((n ** 3, N, N)) + 1j * rng.standard_normal((n ** 3, N, N)))
     g = np.asfortranarray(rng.standard_normal((N, N, N, N)) + 1j * rng.standard_normal((N, N, N, N)))
@@ -162,14 +162,14 @@ def test_mean_field(orc, torch_dev, c0):
     s = SincDVR(n, dx, N)
     rp = s.get_reducedpot(tr, g)
     assert relerr(rp, orc.reducedpot(tr, g)) < 1e-13
-    out = s.mult_reducedpot(2, 4, c0["phi_h"], np.asfortranarray(rp[:, :, 1:4]))
-    assert relerr(out, orc.mult_reducedpot(2, 4, c0["phi_h"], rp[:, :, 1:4])) < 1e-13
+    out = s.mult_reducedpot(2, 4, w21["phi_h"], np.asfortranarray(rp[:, :, 1:4]))
+    assert relerr(out, orc.mult_reducedpot(2, 4, w21["phi_h"], rp[:, :, 1:4])) < 1e-13
 
 
-def test_twoe_matel_c1_41(orc, torch_dev):
+def test_twoe_matel_grid41(orc, torch_dev):
     """41^3 grid, 10 orbitals (55 unique / 100 ordered pairs), hermnorm."""
     from sincdvr import SincDVR, inputs
-    cfg = inputs.CONFIGS["c1_41"]
+    cfg = inputs.CONFIGS["grid41_10orb"]
     n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
     phi = inputs.gaussian_orbitals(n, dx, N, cfg["ngauss"], cfg["seed"])
     say("call_twoe_matel00 n=41 N=10")
@@ -194,7 +194,7 @@ def test_twoe_matel_mixed_charge_carrier(orc, torch_dev, cnorm):
     from the FP32-rounded twiddles and spectrum, which no carrier removes, so
     the matrix elements are only held to the bound.)"""
     from sincdvr import SincDVR, inputs
-    cfg = inputs.CONFIGS["c1_41"]
+    cfg = inputs.CONFIGS["grid41_10orb"]
     n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
     phi = inputs.gaussian_orbitals(n, dx, N, cfg["ngauss"], cfg["seed"], cnorm=cnorm)
     kern = orc.coulomb_kernel(n, dx)
@@ -241,15 +241,15 @@ def test_convolution_count_and_packed_diagonals(orc, torch_dev, cnorm):
                 assert np.all(tr[:, b - 1, b - low].imag == 0.0)
 
 
-def test_twoereduced_entry_and_profiling(torch_dev, c0):
+def test_twoereduced_entry_and_profiling(torch_dev, w21):
     """sdv_twoereduced_dev delivers the same pair potentials as
     call_twoe_matel00, and the event profiler counts one launch per stage."""
     from sincdvr import SincDVR
     torch = torch_dev
-    n, dx = c0["cfg"]["n"], c0["cfg"]["spacing"]
+    n, dx = w21["cfg"]["n"], w21["cfg"]["spacing"]
     say("twoereduced + profile")
     s = SincDVR(n, dx, 4)
-    p = torch.from_numpy(np.ascontiguousarray(c0["phi_h"].T)).cuda()
+    p = torch.from_numpy(np.ascontiguousarray(w21["phi_h"].T)).cuda()
     _, tr = s.call_twoe_matel00(1, 4, p, p)
     s.profile(True)
     tr2 = s.twoereduced(1, 4, p)

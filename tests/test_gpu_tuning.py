@@ -26,7 +26,7 @@ def torch_dev():
 
 
 @pytest.fixture(scope="module")
-def c0(orc):
+def w21(orc):
     from sincdvr import inputs
     n, dx = 21, 0.4
     phi = inputs.gaussian_orbitals(n, dx, 4, 3, 11)
@@ -36,22 +36,22 @@ def c0(orc):
 
 
 @pytest.mark.parametrize("m", [21, 25, 27, 32, 45])
-def test_every_circulant_size_matches_oracle(orc, torch_dev, c0, m):
+def test_every_circulant_size_matches_oracle(orc, torch_dev, w21, m):
     from sincdvr import SincDVR
-    s = SincDVR(c0["n"], c0["dx"], 4)
+    s = SincDVR(w21["n"], w21["dx"], 4)
     say("half edge", m)
     s.set_circulant(m)
     assert s.circulant_edge == 2 * m
-    tm, tr = s.call_twoe_matel00(1, 4, c0["phi"], c0["phi"])
-    assert relerr(tr, c0["otr"]) < 1e-10
-    assert relerr(tm, c0["otm"]) < 1e-10
+    tm, tr = s.call_twoe_matel00(1, 4, w21["phi"], w21["phi"])
+    assert relerr(tr, w21["otr"]) < 1e-10
+    assert relerr(tm, w21["otm"]) < 1e-10
     s.set_circulant(0)
     assert s.circulant_edge == 2 * 21
 
 
-def test_bad_circulant_sizes_rejected(torch_dev, c0):
+def test_bad_circulant_sizes_rejected(torch_dev, w21):
     from sincdvr import SdvError, SincDVR, circulant_sizes
-    s = SincDVR(c0["n"], c0["dx"], 4)
+    s = SincDVR(w21["n"], w21["dx"], 4)
     sizes = circulant_sizes()
     assert sizes == sorted(sizes) and 81 in sizes
     for bad in (16, 22, 1000):   # below n, unsupported, beyond the table
@@ -60,11 +60,11 @@ def test_bad_circulant_sizes_rejected(torch_dev, c0):
     assert s.circulant_edge == 42
 
 
-def test_pair_batch_is_bitwise_neutral(torch_dev, c0):
+def test_pair_batch_is_bitwise_neutral(torch_dev, w21):
     torch = torch_dev
     from sincdvr import SincDVR
-    s = SincDVR(c0["n"], c0["dx"], 4)
-    p = torch.from_numpy(np.ascontiguousarray(c0["phi"].T)).cuda()
+    s = SincDVR(w21["n"], w21["dx"], 4)
+    p = torch.from_numpy(np.ascontiguousarray(w21["phi"].T)).cuda()
     ref = s.twoereduced(1, 4, p)
     for pb in (1, 3, 7, 10):
         s.set_pair_batch(pb)
@@ -73,11 +73,11 @@ def test_pair_batch_is_bitwise_neutral(torch_dev, c0):
         assert torch.equal(got, ref), pb
 
 
-def test_tune_context_and_cache(torch_dev, c0, tmp_path):
+def test_tune_context_and_cache(torch_dev, w21, tmp_path):
     torch = torch_dev
     from sincdvr import SincDVR, autotune
-    s = SincDVR(c0["n"], c0["dx"], 4)
-    p = torch.from_numpy(np.ascontiguousarray(c0["phi"].T)).cuda()
+    s = SincDVR(w21["n"], w21["dx"], 4)
+    p = torch.from_numpy(np.ascontiguousarray(w21["phi"].T)).cuda()
     cache = autotune.TuneCache(str(tmp_path / "tune.json"))
     r = autotune.tune_context(s, p, precision=0, pair_batches=(4, 10), reps=2, cache=cache)
     say("tuned", {k: r[k] for k in ("m", "pair_batch", "ms")})
@@ -85,11 +85,11 @@ def test_tune_context_and_cache(torch_dev, c0, tmp_path):
     assert s.circulant_edge == 2 * r["m"]
     cache.save()
     again = autotune.TuneCache(str(tmp_path / "tune.json"))
-    s2 = SincDVR(c0["n"], c0["dx"], 4)
+    s2 = SincDVR(w21["n"], w21["dx"], 4)
     hit = autotune.tune_context(s2, p, precision=0, cache=again)
     assert hit["m"] == r["m"] and s2.circulant_edge == 2 * r["m"]
     tr = s2.twoereduced(1, 4, p).cpu().numpy().transpose(2, 1, 0)
-    assert relerr(tr, c0["otr"]) < 1e-10
+    assert relerr(tr, w21["otr"]) < 1e-10
 
 
 def test_tune_batch(torch_dev):

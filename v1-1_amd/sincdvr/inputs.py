@@ -98,9 +98,9 @@ def perturb_orbitals(phis, seed, norm=1e-3, cnorm=False):
 
 # Workload configurations (grid edge n, spacing, orbitals, Gaussians each).
 CONFIGS = {
-    "c0_21": dict(n=21, spacing=0.4, norb=4, ngauss=3, seed=0),
-    "c1_41": dict(n=41, spacing=0.3, norb=10, ngauss=4, seed=1000),
-    "c3_81": dict(n=81, spacing=0.25, norb=16, ngauss=6, seed=3000),
+    "grid21_4orb": dict(n=21, spacing=0.4, norb=4, ngauss=3, seed=0),
+    "grid41_10orb": dict(n=41, spacing=0.3, norb=10, ngauss=4, seed=1000),
+    "grid81_16orb": dict(n=81, spacing=0.25, norb=16, ngauss=6, seed=3000),
 }
 
 # Smooth exterior scaling of the 41^3 complex-scaled case: the ramp covers the

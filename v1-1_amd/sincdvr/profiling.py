@@ -42,7 +42,7 @@ def run(precision, steps):
     import torch
 
     from sincdvr import SincDVR, inputs
-    cfg = inputs.CONFIGS["c3_81"]
+    cfg = inputs.CONFIGS["grid81_16orb"]
     n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
     phi = torch.from_numpy(np.ascontiguousarray(
         inputs.gaussian_orbitals(n, dx, N, cfg["ngauss"], cfg["seed"]).T)).cuda()
