@@ -926,7 +926,9 @@ __global__ void __launch_bounds__(kXThreads) k_xfwd_grp(const BSetDev<T>* __rest
                                                    const BItemDev* __restrict__ items,
                                                    const BWork* __restrict__ work, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const BWork wk = work[blockIdx.x];
+  // the work list is chunk-major within a size; each XCD takes one range of
+  // it, as in the single-set x forward (64-set batch: 2-3 % per step)
+  const BWork wk = work[xcd_span(blockIdx.x, gridDim.x)];
   const BItemDev it = items[wk.item];
   body_xfwd<T, P, Q>(batch_view(sets[it.set], it, X), wk.sub, smem_raw);
 }
