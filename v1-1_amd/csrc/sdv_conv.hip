@@ -221,42 +221,55 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
   for (int i = tid; i < M; i += XCfg<P, Q>::XT) tw[i] = a.tw[i];
   const z64* pa = a.pa + (size_t)l0 * n;
   const z64* pb = a.pb + (size_t)l0 * n;
-  // Every load of this thread is issued before the first is consumed (a
-  // loop that waits for each load in turn pays the memory latency ITER
-  // times: 0.85 -> 0.73 ms for the FP64 kernel at 81^3, 136 pairs).
+  // The loads of this thread are issued in NB batches, each batch before
+  // the first of its values is consumed (a loop that waits for each load in
+  // turn pays the memory latency ITER times: 0.85 -> 0.73 ms for the FP64
+  // kernel at 81^3, 136 pairs).  FP32 takes two batches: the FP64 orbital
+  // values of one batch hold half the registers, so four workgroups fit a
+  // CU instead of two (0.49 -> 0.46 ms); FP64 is held to two workgroups by
+  // its LDS and takes one.
   constexpr int ITER = (LA * R + XCfg<P, Q>::XT - 1) / XCfg<P, Q>::XT;  // n <= R
+  constexpr int NB = sizeof(T) == 4 ? 2 : 1;
+  constexpr int IB = (ITER + NB - 1) / NB;
   const int ne = nl * n;
-  z64 ua[ITER], ub[ITER];
-  double gs[ITER];
   const double* gsub = a.gsub ? a.gsub + (size_t)l0 * n : nullptr;
-#pragma unroll
-  for (int i = 0; i < ITER; ++i) {
-    const int e = tid + i * XCfg<P, Q>::XT;
-    gs[i] = 0.0;
-    if (e < ne) {
-      ua[i] = pa[e];
-      if (!a.density_mode) ub[i] = pb[e];
-      if (gsub) gs[i] = gsub[e];
-    }
-  }
   RowCol rc(tid, XCfg<P, Q>::XT, n);
 #pragma unroll
-  for (int i = 0; i < ITER; ++i, rc.next()) {
-    if (tid + i * XCfg<P, Q>::XT < ne) {
-      z64 v = ua[i];
-      if (a.pack) {  // |phi_a|^2 - g + i (|phi_b|^2 - g)
-        const z64 w = ub[i];
-        const z64 da = v * mk(v.re, -v.im), db = w * mk(w.re, -w.im);
-        v = mk(da.re - gs[i], db.re - gs[i]);
-      } else {
-        if (!a.density_mode) {  // SD/coreproject.f90:1057-1065: inspfs10(:,a) * CONJUGATE(inspfs20(:,b))
-          z64 w = ub[i];
-          if (a.conj_b) w.im = -w.im;
-          v = v * w;
-        }
-        v.re -= gs[i];  // neutral remainder of a diagonal pair density (FP32 form)
+  for (int bch = 0; bch < NB; ++bch) {
+    z64 ua[IB], ub[IB];
+    double gs[IB];
+#pragma unroll
+    for (int k = 0; k < IB; ++k) {
+      const int i = bch * IB + k;
+      const int e = tid + i * XCfg<P, Q>::XT;
+      gs[k] = 0.0;
+      if (i < ITER && e < ne) {
+        ua[k] = pa[e];
+        if (!a.density_mode) ub[k] = pb[e];
+        if (gsub) gs[k] = gsub[e];
       }
-      buf[rc.row * pitch + rc.col] = cvt<T>(v);
+    }
+#pragma unroll
+    for (int k = 0; k < IB; ++k) {
+      const int i = bch * IB + k;
+      if (i >= ITER) break;
+      if (tid + i * XCfg<P, Q>::XT < ne) {
+        z64 v = ua[k];
+        if (a.pack) {  // |phi_a|^2 - g + i (|phi_b|^2 - g)
+          const z64 w = ub[k];
+          const z64 da = v * mk(v.re, -v.im), db = w * mk(w.re, -w.im);
+          v = mk(da.re - gs[k], db.re - gs[k]);
+        } else {
+          if (!a.density_mode) {  // SD/coreproject.f90:1057-1065: inspfs10(:,a) * CONJUGATE(inspfs20(:,b))
+            z64 w = ub[k];
+            if (a.conj_b) w.im = -w.im;
+            v = v * w;
+          }
+          v.re -= gs[k];  // neutral remainder of a diagonal pair density (FP32 form)
+        }
+        buf[rc.row * pitch + rc.col] = cvt<T>(v);
+      }
+      rc.next();
     }
   }
   __syncthreads();
