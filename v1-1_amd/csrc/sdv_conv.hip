@@ -627,8 +627,12 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 template <typename T, int P, int Q, bool FULL>
 __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_yz<T, P, Q, false, KArgs<T>, FULL>(a, a.X + ((size_t)blockIdx.y * 2 * (P * Q) + blockIdx.x) * a.ld,
-                                          blockIdx.x, smem_raw);
+  // logical order plane-major, pairs fastest, one range per XCD: an XCD
+  // works on a few x-frequency planes and keeps their spectrum slices in its
+  // L2 (FP32 81^3: 1.60 -> 1.57 ms; FP64 unchanged)
+  const unsigned L = xcd_span(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const unsigned plane = L / gridDim.y, pair = L % gridDim.y;
+  body_yz<T, P, Q, false, KArgs<T>, FULL>(a, a.X + ((size_t)pair * 2 * (P * Q) + plane) * a.ld, plane, smem_raw);
 }
 
 // ---------------------------------------------------------------- x inverse
