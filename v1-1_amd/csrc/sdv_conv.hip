@@ -149,7 +149,8 @@ struct XCfg {
 // A group could take LPG lines one after the other (smaller workgroups, two
 // per CU for FP32's 54 KB of LDS at m = 81); measured at 81^3 that is 1.4x
 // slower in FP32 than one line per group at 12 waves, and 1.2x slower in FP64
-// (6 waves, no register spill, against 32 bytes per lane of spill at 12), so
+// (6 waves, no register spill, against 32 bytes per lane of spill at 12 with
+// the odd twiddles in registers), so
 // LPG = 1.  Static wave priorities that differ between the three waves of a
 // SIMD (to stagger their LDS and VALU phases) change nothing measurable.
 // LDS after the plane.  FP32: the kernel stages its x-frequency plane's slice
@@ -162,7 +163,7 @@ struct XCfg {
 #define SDV_YZ_KHAT_LDS 1
 #endif
 #ifndef SDV_YZ_TW_REGS
-#define SDV_YZ_TW_REGS 1
+#define SDV_YZ_TW_REGS 0
 #endif
 constexpr size_t yz_base_bytes(int R, int es) {
   return (size_t)(3 * R + (R & 1) + R * (R | 1)) * es;  // tables (2R + R + pad) and plane, bytes
@@ -401,10 +402,12 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   }
   cplx<T> o0[KEEP ? LPG : 1][P];  // cy = 0 output part (FP32)
 
-  // Odd-channel twiddle w_M^(j1 Q + li) = w_M^li * w_2P^j1.  FP64: the lane's
-  // w_M^li (one LDS read) times the compile-time w_2P^j1, which saves P - 1
-  // 16-byte LDS reads per use at one complex multiply each; FP32 reads the
-  // correctly rounded table entry.
+  // Odd-channel twiddle w_M^(j1 Q + li) = w_M^li * w_2P^j1: the correctly
+  // rounded table entry from LDS.  SDV_YZ_TW_REGS=1 forms it in FP64 as the
+  // lane's w_M^li (one LDS read) times the compile-time w_2P^j1, saving P - 1
+  // 16-byte LDS reads per use at one complex multiply each; its registers
+  // raise the kernel's spill from 20 to 32 bytes per lane and it runs 3.5 %
+  // slower at 81^3, so both precisions read the table.
   constexpr bool TW_REG = sizeof(T) == 8 && SDV_YZ_TW_REGS;
   auto odd_base = [&]() -> cplx<T> { return TW_REG ? tw[li] : mk((T)0, (T)0); };
   auto odd_tw = [&](const cplx<T>& wb, int j1) -> cplx<T> {
