@@ -136,10 +136,10 @@ __device__ __forceinline__ unsigned xcd_span(unsigned L, unsigned total) {
   return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
 }
 
-template <int P, int Q>
+template <int P, int Q, int XT_ = kXThreads>
 struct XCfg {
   static constexpr int MX = P > Q ? P : Q;
-  static constexpr int XT = kXThreads;  // workgroup size of the x kernels
+  static constexpr int XT = XT_;        // workgroup size of the x kernels
   static constexpr int LA = XT / MX;    // x-lines per workgroup (56 at m = 81: 0.9 KB runs per plane row)
 };
 
@@ -193,9 +193,9 @@ struct YZCfg {
 };
 
 // ---------------------------------------------------------------- x forward
-template <typename T, int P, int Q>
+template <typename T, int P, int Q, int XT = kXThreads>
 __device__ __forceinline__ void xfwd_step2(const ItemView<T>& a, const cplx<T>* buf, int l0, int nl, int c) {
-  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1;
+  constexpr int R = P * Q, LA = XCfg<P, Q, XT>::LA, pitch = R | 1;
   const int tid = threadIdx.x;
   const int line = tid % LA, k1 = tid / LA;
   if (k1 < P && line < nl) {
@@ -209,17 +209,17 @@ __device__ __forceinline__ void xfwd_step2(const ItemView<T>& a, const cplx<T>* 
   }
 }
 
-// x forward of lines [chunk*LA, chunk*LA + LA) of one pair (kXThreads threads).
-template <typename T, int P, int Q>
+// x forward of lines [chunk*LA, chunk*LA + LA) of one pair (XT threads).
+template <typename T, int P, int Q, int XT = kXThreads>
 __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsigned char* smem_raw) {
-  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
+  constexpr int R = P * Q, LA = XCfg<P, Q, XT>::LA, pitch = R | 1, M = 2 * R;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + M;
   const int tid = threadIdx.x;
   const int n = a.n;
   const int l0 = chunk * LA;
   const int nl = min(LA, a.n2 - l0);
-  for (int i = tid; i < M; i += XCfg<P, Q>::XT) tw[i] = a.tw[i];
+  for (int i = tid; i < M; i += XCfg<P, Q, XT>::XT) tw[i] = a.tw[i];
   const z64* pa = a.pa + (size_t)l0 * n;
   const z64* pb = a.pb + (size_t)l0 * n;
   // The loads of this thread are issued in NB batches, each batch before
@@ -229,12 +229,12 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
   // values of one batch hold half the registers, so four workgroups fit a
   // CU instead of two (0.49 -> 0.46 ms); FP64 is held to two workgroups by
   // its LDS and takes one.
-  constexpr int ITER = (LA * R + XCfg<P, Q>::XT - 1) / XCfg<P, Q>::XT;  // n <= R
+  constexpr int ITER = (LA * R + XCfg<P, Q, XT>::XT - 1) / XCfg<P, Q, XT>::XT;  // n <= R
   constexpr int NB = sizeof(T) == 4 ? 2 : 1;
   constexpr int IB = (ITER + NB - 1) / NB;
   const int ne = nl * n;
   const double* gsub = a.gsub ? a.gsub + (size_t)l0 * n : nullptr;
-  RowCol rc(tid, XCfg<P, Q>::XT, n);
+  RowCol rc(tid, XCfg<P, Q, XT>::XT, n);
 #pragma unroll
   for (int bch = 0; bch < NB; ++bch) {
     z64 ua[IB], ub[IB];
@@ -242,7 +242,7 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
 #pragma unroll
     for (int k = 0; k < IB; ++k) {
       const int i = bch * IB + k;
-      const int e = tid + i * XCfg<P, Q>::XT;
+      const int e = tid + i * XCfg<P, Q, XT>::XT;
       gs[k] = 0.0;
       if (i < ITER && e < ne) {
         ua[k] = pa[e];
@@ -254,7 +254,7 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
     for (int k = 0; k < IB; ++k) {
       const int i = bch * IB + k;
       if (i >= ITER) break;
-      if (tid + i * XCfg<P, Q>::XT < ne) {
+      if (tid + i * XCfg<P, Q, XT>::XT < ne) {
         z64 v = ua[k];
         if (a.pack) {  // |phi_a|^2 - g + i (|phi_b|^2 - g)
           const z64 w = ub[k];
@@ -296,18 +296,29 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
     }
   }
   __syncthreads();
-  xfwd_step2<T, P, Q>(a, buf, l0, nl, 0);
+  xfwd_step2<T, P, Q, XT>(a, buf, l0, nl, 0);
   __syncthreads();
   if (act) {
 #pragma unroll
     for (int k1 = 0; k1 < P; ++k1) buf[line * pitch + k1 * Q + j2] = b[k1];
   }
   __syncthreads();
-  xfwd_step2<T, P, Q>(a, buf, l0, nl, 1);
+  xfwd_step2<T, P, Q, XT>(a, buf, l0, nl, 1);
 }
 
+// Workgroup sizes of the single-set x kernels per precision, measured at
+// 81^3: the FP32 x forward at 288 threads (32 lines, 0.47 -> 0.39 ms), the
+// FP64 x inverse at 360 (40 lines, 0.84 -> 0.81 ms), the other two at 512.
+// The batched kernels share one line-chunk list between their x forward and
+// x inverse and keep 512; the arithmetic of a line does not depend on it.
+template <typename T>
+struct XThreads {
+  static constexpr int fwd = sizeof(T) == 4 ? 288 : kXThreads;
+  static constexpr int inv = sizeof(T) == 8 ? 360 : kXThreads;
+};
+
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(kXThreads) k_xfwd(KArgs<T> a) {
+__global__ void __launch_bounds__(XThreads<T>::fwd) k_xfwd(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // pairs run fastest in the logical order, so the workgroups resident at
   // one time work on the same line chunk of many pairs, and each XCD takes
@@ -317,7 +328,7 @@ __global__ void __launch_bounds__(kXThreads) k_xfwd(KArgs<T> a) {
   // orbitals 8 times: 1.08 GB per launch at 81^3 with 16 orbitals;
   // 0.60 -> 0.57 ms FP64, FP32 unchanged)
   const unsigned L = xcd_span(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-  body_xfwd<T, P, Q>(make_view(a, L % gridDim.x, P * Q), L / gridDim.x, smem_raw);
+  body_xfwd<T, P, Q, XThreads<T>::fwd>(make_view(a, L % gridDim.x, P * Q), L / gridDim.x, smem_raw);
 }
 
 // ------------------------------------------------------------- y/z planes
@@ -640,17 +651,17 @@ __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz(KArgs<T> a) {
 
 // ---------------------------------------------------------------- x inverse
 // x inverse of lines [chunk*LA, chunk*LA + LA) of one pair and the output
-// scatter (kXThreads threads).
-template <typename T, int P, int Q>
+// scatter (XT threads).
+template <typename T, int P, int Q, int XT = kXThreads>
 __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsigned char* smem_raw) {
-  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
+  constexpr int R = P * Q, LA = XCfg<P, Q, XT>::LA, pitch = R | 1, M = 2 * R;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + M;
   const int tid = threadIdx.x;
   const int n = a.n;
   const int l0 = chunk * LA;
   const int nl = min(LA, a.n2 - l0);
-  for (int i = tid; i < M; i += XCfg<P, Q>::XT) tw[i] = a.tw[i];
+  for (int i = tid; i < M; i += XCfg<P, Q, XT>::XT) tw[i] = a.tw[i];
   const int line = tid % LA, jj = tid / LA;
   cplx<T> acc[P];
 #pragma unroll
@@ -663,20 +674,20 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
     // FP32 at 81^3, 136 pairs).  Row and column come from the element index:
     // stepping a row/column pair through the unrolled loop kept the loaded
     // values in scratch memory instead of registers.
-    constexpr int ITER = (LA * R + XCfg<P, Q>::XT - 1) / XCfg<P, Q>::XT;
+    constexpr int ITER = (LA * R + XCfg<P, Q, XT>::XT - 1) / XCfg<P, Q, XT>::XT;
     const int ne = R * nl;
     cplx<T> ld[ITER];
 #pragma unroll
     for (int i = 0; i < ITER; ++i) {
-      const int e = tid + i * XCfg<P, Q>::XT;
+      const int e = tid + i * XCfg<P, Q, XT>::XT;
       const int row = e / nl;
       ld[i] = e < ne ? src[(size_t)row * a.ld + (e - row * nl)] : mk((T)0, (T)0);
     }
     {
-      RowCol rc(tid, XCfg<P, Q>::XT, nl);
+      RowCol rc(tid, XCfg<P, Q, XT>::XT, nl);
 #pragma unroll
       for (int i = 0; i < ITER; ++i, rc.next())
-        if (tid + i * XCfg<P, Q>::XT < ne) buf[rc.col * pitch + rc.row] = ld[i];
+        if (tid + i * XCfg<P, Q, XT>::XT < ne) buf[rc.col * pitch + rc.row] = ld[i];
     }
     __syncthreads();
     if (jj < P && line < nl) {  // inverse step 1: task (line, k1)
@@ -710,8 +721,8 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
   z64* o = a.out + (size_t)l0 * n;
   z64* om = a.outm ? a.outm + (size_t)l0 * n : nullptr;
   const double* vsub = a.vsub ? a.vsub + (size_t)l0 * n : nullptr;
-  RowCol rc(tid, XCfg<P, Q>::XT, n);
-  for (int e = tid; e < nl * n; e += XCfg<P, Q>::XT, rc.next()) {
+  RowCol rc(tid, XCfg<P, Q, XT>::XT, n);
+  for (int e = tid; e < nl * n; e += XCfg<P, Q, XT>::XT, rc.next()) {
     const cplx<T> v = buf[rc.row * pitch + rc.col];
     z64 d = mk((double)v.re, (double)v.im);
     if (vsub) {  // K*g of the charge carrier, in FP64
@@ -732,15 +743,17 @@ __device__ __forceinline__ void body_xinv(const ItemView<T>& a, int chunk, unsig
 }
 
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(kXThreads) k_xinv(KArgs<T> a) {
+__global__ void __launch_bounds__(XThreads<T>::inv) k_xinv(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_xinv<T, P, Q>(make_view(a, blockIdx.y, P * Q), blockIdx.x, smem_raw);
+  body_xinv<T, P, Q, XThreads<T>::inv>(make_view(a, blockIdx.y, P * Q), blockIdx.x, smem_raw);
 }
 
 // ---------------------------------------------------------------- dispatch
 template <typename T, int P, int Q>
 static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, Prof* prof) {
-  constexpr int R = P * Q, LA = XCfg<P, Q>::LA, pitch = R | 1, M = 2 * R;
+  constexpr int R = P * Q, pitch = R | 1, M = 2 * R;
+  constexpr int TF = XThreads<T>::fwd, TI = XThreads<T>::inv;
+  constexpr int LAF = XCfg<P, Q, TF>::LA, LAI = XCfg<P, Q, TI>::LA;
   KArgs<T> a;
   a.n = h.n;
   a.n2 = h.n * h.n;
@@ -760,7 +773,8 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   const bool carrier = sizeof(T) == 4 && !h.density_mode && h.phi1 == h.phi2 && s.gsub && s.vsub;
   a.gsub = carrier ? s.gsub : nullptr;
   a.vsub = carrier ? s.vsub : nullptr;
-  const size_t shX = (size_t)(M + LA * pitch) * sizeof(cplx<T>);
+  const size_t shXF = (size_t)(M + LAF * pitch) * sizeof(cplx<T>);
+  const size_t shXI = (size_t)(M + LAI * pitch) * sizeof(cplx<T>);
   a.lds_tw2 = M;
   a.lds_y = M + R + ((M + R) & 1);  // keep the workspaces 16-byte aligned for FP32
   const size_t shB = yz_lds_bytes(R, (int)sizeof(cplx<T>));
@@ -769,13 +783,14 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   const bool full = h.n == R;
   const void* kyz = full ? (const void*)k_yz<T, P, Q, true> : (const void*)k_yz<T, P, Q, false>;
   SDV_HIP_CHECK(hipFuncSetAttribute(kyz, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shB));
-  SDV_REQUIRE(shX <= 160 * 1024, SDV_ERR_UNSUPPORTED, "x lines do not fit in LDS");
-  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xfwd<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shX));
-  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xinv<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shX));
-  dim3 gX((a.n2 + LA - 1) / LA, h.npairs);
+  SDV_REQUIRE(shXF <= 160 * 1024 && shXI <= 160 * 1024, SDV_ERR_UNSUPPORTED, "x lines do not fit in LDS");
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xfwd<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shXF));
+  SDV_HIP_CHECK(hipFuncSetAttribute((const void*)k_xinv<T, P, Q>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shXI));
+  dim3 gXF(h.npairs, (a.n2 + LAF - 1) / LAF);  // pairs fastest (renumbered by XCD inside)
+  dim3 gXI((a.n2 + LAI - 1) / LAI, h.npairs);
   dim3 gB(2 * R, h.npairs);
   hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xfwd<T, P, Q>), dim3(h.npairs, gX.x), dim3(kXThreads), shX, st, a);
+  hipLaunchKernelGGL((k_xfwd<T, P, Q>), gXF, dim3(TF), shXF, st, a);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
@@ -786,7 +801,7 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xinv<T, P, Q>), gX, dim3(kXThreads), shX, st, a);
+  hipLaunchKernelGGL((k_xinv<T, P, Q>), gXI, dim3(TI), shXI, st, a);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XINV, e0, st);
   return SDV_OK;
