@@ -306,15 +306,17 @@ __device__ __forceinline__ void body_xfwd(const ItemView<T>& a, int chunk, unsig
   xfwd_step2<T, P, Q, XT>(a, buf, l0, nl, 1);
 }
 
-// Workgroup sizes of the single-set x kernels per precision, measured at
-// 81^3: the FP32 x forward at 288 threads (32 lines, 0.47 -> 0.39 ms), the
-// FP64 x inverse at 360 (40 lines, 0.84 -> 0.81 ms), the other two at 512.
-// The batched kernels share one line-chunk list between their x forward and
-// x inverse and keep 512; the arithmetic of a line does not depend on it.
+// Workgroup sizes of the x kernels per precision, measured at 81^3: the FP32
+// x forward at 288 threads (32 lines, 0.47 -> 0.39 ms), the FP64 x inverse at
+// 360 (40 lines, 0.84 -> 0.81 ms), the other two at 512 (the FP32 x inverse
+// runs the same at 288).  The batched kernels share one line-chunk list
+// between their x forward and x inverse: 288 in FP32, 512 in FP64.  The
+// arithmetic of a line does not depend on the size.
 template <typename T>
 struct XThreads {
   static constexpr int fwd = sizeof(T) == 4 ? 288 : kXThreads;
   static constexpr int inv = sizeof(T) == 8 ? 360 : kXThreads;
+  static constexpr int batch = sizeof(T) == 4 ? 288 : kXThreads;  // both batched x kernels
 };
 
 template <typename T, int P, int Q>
@@ -875,7 +877,7 @@ __device__ __forceinline__ ItemView<T> batch_view(const BSetDev<T>& s, const BIt
 // allocation and the kernel needs only the largest.
 template <typename T, int P, int Q>
 __device__ __noinline__ void call_xfwd(const ItemView<T>* v, int sub, unsigned char* smem) {
-  body_xfwd<T, P, Q>(*v, sub, smem);
+  body_xfwd<T, P, Q, XThreads<T>::batch>(*v, sub, smem);
 }
 template <typename T, int P, int Q>
 __device__ __noinline__ void call_yz(const ItemView<T>* v, int sub, unsigned char* smem) {
@@ -886,11 +888,11 @@ __device__ __noinline__ void call_yz(const ItemView<T>* v, int sub, unsigned cha
 }
 template <typename T, int P, int Q>
 __device__ __noinline__ void call_xinv(const ItemView<T>* v, int sub, unsigned char* smem) {
-  body_xinv<T, P, Q>(*v, sub, smem);
+  body_xinv<T, P, Q, XThreads<T>::batch>(*v, sub, smem);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kXThreads) k_xfwd_multi(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__(XThreads<T>::batch) k_xfwd_multi(const BSetDev<T>* __restrict__ sets,
                                                      const BItemDev* __restrict__ items,
                                                      const BWork* __restrict__ work, int nwork, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -924,7 +926,7 @@ __global__ void __launch_bounds__(768) k_yz_multi(const BSetDev<T>* __restrict__
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kXThreads) k_xinv_multi(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__(XThreads<T>::batch) k_xinv_multi(const BSetDev<T>* __restrict__ sets,
                                                      const BItemDev* __restrict__ items,
                                                      const BWork* __restrict__ work, int nwork, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -941,8 +943,9 @@ __global__ void __launch_bounds__(kXThreads) k_xinv_multi(const BSetDev<T>* __re
 }
 
 // SDV_SIZE_SWITCH is itself the switch statement; unsupported m gives 0.
-static int lines_per_block(int m) {
-#define SDV_LA(P_, Q_) return XCfg<P_, Q_>::LA
+static int lines_per_block(int m, int precision) {
+#define SDV_LA(P_, Q_) \
+  return precision ? XCfg<P_, Q_, XThreads<float>::batch>::LA : XCfg<P_, Q_, XThreads<double>::batch>::LA
   SDV_SIZE_SWITCH(m, SDV_LA)
 #undef SDV_LA
   return 0;
@@ -957,7 +960,7 @@ static int yz_threads(int m, int precision) {
 // Grouped schedule: one work entry per workgroup, block shape and LDS of
 // this circulant size (as the single-set kernels).
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(kXThreads) k_xfwd_grp(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__(XThreads<T>::batch) k_xfwd_grp(const BSetDev<T>* __restrict__ sets,
                                                    const BItemDev* __restrict__ items,
                                                    const BWork* __restrict__ work, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -965,7 +968,7 @@ __global__ void __launch_bounds__(kXThreads) k_xfwd_grp(const BSetDev<T>* __rest
   // it, as in the single-set x forward (64-set batch: 2-3 % per step)
   const BWork wk = work[xcd_span(blockIdx.x, gridDim.x)];
   const BItemDev it = items[wk.item];
-  body_xfwd<T, P, Q>(batch_view(sets[it.set], it, X), wk.sub, smem_raw);
+  body_xfwd<T, P, Q, XThreads<T>::batch>(batch_view(sets[it.set], it, X), wk.sub, smem_raw);
 }
 
 template <typename T, int P, int Q, bool FULL>
@@ -980,18 +983,20 @@ __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz_grp(const BSetDev<T
 }
 
 template <typename T, int P, int Q>
-__global__ void __launch_bounds__(kXThreads) k_xinv_grp(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__(XThreads<T>::batch) k_xinv_grp(const BSetDev<T>* __restrict__ sets,
                                                    const BItemDev* __restrict__ items,
                                                    const BWork* __restrict__ work, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const BWork wk = work[blockIdx.x];
   const BItemDev it = items[wk.item];
-  body_xinv<T, P, Q>(batch_view(sets[it.set], it, X), wk.sub, smem_raw);
+  body_xinv<T, P, Q, XThreads<T>::batch>(batch_view(sets[it.set], it, X), wk.sub, smem_raw);
 }
 
 // LDS of the x kernels and of the y/z kernel for half-edge m (elements of
 // cplx<T>).
-static size_t lds_x_elems(int m) { return (size_t)2 * m + (size_t)lines_per_block(m) * (m | 1); }
+static size_t lds_x_elems(int m, int precision) {
+  return (size_t)2 * m + (size_t)lines_per_block(m, precision) * (m | 1);
+}
 static size_t lds_yz_elems(int m, int elem_bytes) {
   return (yz_lds_bytes(m, elem_bytes) + elem_bytes - 1) / elem_bytes;
 }
@@ -1017,13 +1022,13 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
     const BatchSetHost& s = sets[si];
     const int n = s.spec->g.n, m = s.spec->g.m;
     const int key = 2 * m + (n == m ? 1 : 0);
-    const int LA = lines_per_block(m);
+    const int LA = lines_per_block(m, precision);
     if (LA <= 0 || yz_threads(m, precision) <= 0 || m >= 128) {  // every context's m comes from choose_geom
       bp.items.clear();
       bp.grid_x = bp.grid_y = 0;
       return bp;
     }
-    shx = std::max(shx, lds_x_elems(m) * es);
+    shx = std::max(shx, lds_x_elems(m, precision) * es);
     shy = std::max(shy, lds_yz_elems(m, (int)es) * es);
     nty = std::max(nty, yz_threads(m, precision));
     for (const PairItem& p : *s.items) {
@@ -1072,7 +1077,7 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
       g.wy0 = (int)i;
       while (i < wy.size() && wy[i].key == key) ++i;
       g.nwy = (int)i - g.wy0;
-      g.shx = lds_x_elems(g.m) * es;
+      g.shx = lds_x_elems(g.m, precision) * es;
       g.shy = lds_yz_elems(g.m, (int)es) * es;
       bp.groups.push_back(g);
     }
@@ -1111,7 +1116,7 @@ static int batch_launch_fused(const BatchPlan& bp, const BatchDev& d, hipStream_
                                     (int)bp.shx));
   const BSetDev<T>* sets = (const BSetDev<T>*)d.sets;
   hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL(k_xfwd_multi<T>, dim3(bp.grid_x), dim3(kXThreads), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
+  hipLaunchKernelGGL(k_xfwd_multi<T>, dim3(bp.grid_x), dim3(XThreads<T>::batch), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
                      (cplx<T>*)d.X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
@@ -1121,7 +1126,7 @@ static int batch_launch_fused(const BatchPlan& bp, const BatchDev& d, hipStream_
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL(k_xinv_multi<T>, dim3(bp.grid_x), dim3(kXThreads), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
+  hipLaunchKernelGGL(k_xinv_multi<T>, dim3(bp.grid_x), dim3(XThreads<T>::batch), bp.shx, st, sets, d.items, d.wx, (int)bp.wx.size(),
                      (cplx<T>*)d.X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XINV, e0, st);
@@ -1140,7 +1145,7 @@ static int launch_group(const BatchGroup& g, const BatchDev& d, hipStream_t st, 
   const BSetDev<T>* sets = (const BSetDev<T>*)d.sets;
   cplx<T>* X = (cplx<T>*)d.X;
   hipEvent_t e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xfwd_grp<T, P, Q>), dim3(g.nwx), dim3(kXThreads), g.shx, st, sets, d.items, d.wx + g.wx0, X);
+  hipLaunchKernelGGL((k_xfwd_grp<T, P, Q>), dim3(g.nwx), dim3(XThreads<T>::batch), g.shx, st, sets, d.items, d.wx + g.wx0, X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XFWD, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
@@ -1153,7 +1158,7 @@ static int launch_group(const BatchGroup& g, const BatchDev& d, hipStream_t st, 
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_YZ, e0, st);
   e0 = prof ? prof->begin(st) : nullptr;
-  hipLaunchKernelGGL((k_xinv_grp<T, P, Q>), dim3(g.nwx), dim3(kXThreads), g.shx, st, sets, d.items, d.wx + g.wx0, X);
+  hipLaunchKernelGGL((k_xinv_grp<T, P, Q>), dim3(g.nwx), dim3(XThreads<T>::batch), g.shx, st, sets, d.items, d.wx + g.wx0, X);
   SDV_HIP_CHECK(hipGetLastError());
   if (prof) prof->end(SDV_STAGE_XINV, e0, st);
   return SDV_OK;
