@@ -149,7 +149,11 @@ def bench_serving(torch, quick):
     nprop = 32 if quick else 256
     descs = serving.propagations(nprop=nprop)
     ctxs, props = serving.build_propagations(descs, steps=2)
-    budget, tick = 48 << 30, 4 << 30
+    # 1 GiB ticks: the same capacity as 4 GiB ticks for these requests, with a
+    # shorter wait for a request that arrives while a tick runs (Coulomb p99
+    # at 70 % load over repeated runs: mixed 8.9-13.2 ms with 4 GiB ticks,
+    # 7.2-9.7 ms with 1 GiB; FP64 14.0-14.6 ms -> 11.5-11.8 ms)
+    budget, tick = 48 << 30, 1 << 30
     res = {"propagations": nprop, "grid_edges": list(serving.SERVING_EDGES), "orbitals": "6..16",
            "hbm_budget_gb": budget / 2 ** 30, "tick_work_gb": tick / 2 ** 30,
            "requests": "alternating Coulomb (all pair potentials) and kinetic apply per propagation",
