@@ -84,6 +84,18 @@ int sdv_nuclear_potential(sdv_ctx* ctx, int numcenters, const int* centershift, 
 int sdv_op_tinv(sdv_ctx* ctx, const double* twoeden03, double* twoereduced, int allsize, int circsize);
 int sdv_op_tinv_dev(sdv_ctx* ctx, const void* twoeden03, void* twoereduced, int allsize, int circsize,
                     void* stream);
+/* Allocation-free op_tinv.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * sdv_op_tinv_dev (bitwise: the same kernels per density); it calls no
+ * hipMalloc/hipFree.  Its work memory is one library-owned 128 MiB device
+ * array per GPU, held until the process ends; the densities are convolved in
+ * chunks that fit it, and a grid whose single convolution does not fit is
+ * refused with SDV_ERR_UNSUPPORTED (every supported grid fits: 17 MB per
+ * density at 81^3 in FP64).  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (different streams or host threads) race. */
+int sdv_op_tinv_noalloc_dev(sdv_ctx* ctx, const void* twoeden03, void* twoereduced, int allsize, int circsize,
+                            void* stream);
 
 /* call_twoe_matel00 (SD/coreproject.f90:945-955):
  *   inspfs10, inspfs20 (totpoints, numspf) in;

@@ -212,6 +212,8 @@ void sdv::spectrum_release(Spectrum* s) {
     }
 }
 
+static int scratch_ptr(void** out);
+
 extern "C" {
 
 void sdv_config_default(sdv_config* c) {
@@ -248,6 +250,10 @@ int sdv_init(const sdv_config* cfg, sdv_ctx** out) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
     set_error(std::string("library is built for gfx950, device is ") + prop.gcnArchName);
     return SDV_ERR_NODEVICE;
+  }
+  {
+    void* sp = nullptr;  // per-device address of the allocation-free entry point's scratch
+    SDV_TRY(scratch_ptr(&sp));
   }
   sdv_ctx* c = new sdv_ctx();
   c->cfg = *cfg;
@@ -577,6 +583,104 @@ int sdv_op_tinv_dev(sdv_ctx* c, const void* den, void* red, int allsize, int cir
   IdxPlan* pl = nullptr;
   SDV_TRY(density_plan(c, allsize, &pl));
   return run_pairs(c, *pl, den, den, 1, red, st);
+}
+
+}  // extern "C"
+
+// ---- allocation-free op_tinv
+// The device memory of sdv_op_tinv_noalloc_dev: the pair items of one chunk
+// at the front, the chunk's convolution intermediates behind them.  One copy
+// per device (the array below), held until the process ends.  Host code never
+// passes the array's name to a kernel: the name is a host-side stand-in, and
+// each device's address comes from hipGetSymbolAddress (scratch_ptr).
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+constexpr int kNoAllocItems = 4096;  // PairItem slots at the front (80 KB)
+constexpr size_t kNoAllocItemBytes = ((size_t)kNoAllocItems * sizeof(PairItem) + 255) & ~(size_t)255;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The current device's address of `scratch`, looked up once per device (at
+// sdv_init, else at the entry point's first call on that device).
+static int scratch_ptr(void** out) {
+  int dev = 0;
+  SDV_HIP_CHECK(hipGetDevice(&dev));
+  SDV_REQUIRE(dev >= 0 && dev < kMaxDevices, SDV_ERR_UNSUPPORTED, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return SDV_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SDV_OK;
+}
+
+__global__ void k_density_items(PairItem* items, int p0, int np) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < np) items[j] = PairItem{p0 + j, 0, p0 + j, -1, 0};
+}
+
+__global__ void k_zero_z64(z64* p, long long count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = mk(0.0, 0.0);
+}
+
+extern "C" {
+
+int sdv_op_tinv_noalloc_dev(sdv_ctx* c, const void* den, void* red, int allsize, int circsize, void* stream) {
+  SDV_REQUIRE(c && den && red, SDV_ERR_ARG, "null argument");
+  SDV_REQUIRE(allsize >= 0 && circsize > 0 && allsize % circsize == 0, SDV_ERR_ARG,
+              "SIZE ERROR OP_TINV: allsize must be a multiple of circsize");
+  hipStream_t st = (hipStream_t)stream;
+  const int dbg = c->cfg.debugflag;
+  if (dbg == 4040 || dbg == 4242 || dbg == 4343) {  // SD/coreproject.f90:260-263
+    const long long cnt = (long long)c->n3 * allsize;
+    if (cnt > 0) {
+      hipLaunchKernelGGL(k_zero_z64, dim3((unsigned)std::min<long long>((cnt + 255) / 256, 4096)), dim3(256), 0, st,
+                         (z64*)red, cnt);
+      SDV_HIP_CHECK(hipGetLastError());
+    }
+    return SDV_OK;
+  }
+  if (allsize == 0) return SDV_OK;
+  void* base = nullptr;
+  SDV_TRY(scratch_ptr(&base));
+  const size_t per = conv_work_bytes(c->spec->g, 1, c->precision);
+  SDV_REQUIRE(per <= kNoAllocScratchBytes - kNoAllocItemBytes, SDV_ERR_UNSUPPORTED,
+              "one convolution's work buffer exceeds the 128 MiB allocation-free scratch");
+  const int chunk = (int)std::min<size_t>((size_t)kNoAllocItems, (kNoAllocScratchBytes - kNoAllocItemBytes) / per);
+  PairItem* items = (PairItem*)base;
+  void* work = (char*)base + kNoAllocItemBytes;
+  for (int p0 = 0; p0 < allsize; p0 += chunk) {
+    const int np = std::min(chunk, allsize - p0);
+    // Items are rewritten per chunk; the stream orders this after the previous
+    // chunk's kernels, which read them.
+    hipLaunchKernelGGL(k_density_items, dim3((np + 255) / 256), dim3(256), 0, st, items, p0, np);
+    SDV_HIP_CHECK(hipGetLastError());
+    ConvArgsHost a;
+    a.n = c->n;
+    a.npairs = np;
+    a.phi1 = den;
+    a.phi2 = den;
+    a.density_mode = 1;
+    a.conj_b = 0;
+    a.mirror_conj = c->cfg.cnormflag ? 0 : 1;
+    a.pairs_dev = items;
+    a.work = work;
+    a.out = red;
+    SDV_TRY(conv_launch(*c->spec, a, c->precision, st, nullptr));
+  }
+  return SDV_OK;
 }
 
 static int check_twoe_args(sdv_ctx* c, int low, int high) {

@@ -15,7 +15,7 @@ ERRORS = {-1: "SDV_ERR_ARG", -2: "SDV_ERR_HIP", -3: "SDV_ERR_UNSUPPORTED", -4: "
 # Every symbol include/sincdvr.h declares.
 EXPORTS = [
     "sdv_config_default", "sdv_init", "sdv_free", "sdv_last_error", "sdv_set_precision",
-    "sdv_op_tinv", "sdv_op_tinv_dev",
+    "sdv_op_tinv", "sdv_op_tinv_dev", "sdv_op_tinv_noalloc_dev",
     "sdv_call_twoe_matel00", "sdv_call_twoe_matel00_dev",
     "sdv_mult_ke", "sdv_mult_ke_dev",
     "sdv_mult_reducedpot", "sdv_mult_reducedpot_dev",
@@ -94,6 +94,7 @@ def load():
     lib.sdv_set_precision.argtypes = [vp, i]
     lib.sdv_op_tinv.argtypes = [vp, vp, vp, i, i]
     lib.sdv_op_tinv_dev.argtypes = [vp, vp, vp, i, i, vp]
+    lib.sdv_op_tinv_noalloc_dev.argtypes = [vp, vp, vp, i, i, vp]
     lib.sdv_call_twoe_matel00.argtypes = [vp, i, i, vp, vp, vp, vp]
     lib.sdv_call_twoe_matel00_dev.argtypes = [vp, i, i, vp, vp, vp, vp, vp]
     lib.sdv_mult_ke.argtypes = [vp, vp, vp, i]
