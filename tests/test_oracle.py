@@ -133,6 +133,25 @@ def test_coulomb_kernel_is_inverse_laplacian(orc):
     assert np.abs(lap[c - 1:c + 2, c - 1:c + 2, c - 1:c + 2]).sum() - abs(lap[c, c, c]) < 0.2
 
 
+def test_coulomb_kernel_gaussian_charge(orc):
+    """Physical pin of GETINVERSE + kernel placement: the grid potential of a
+    normalised Gaussian charge (alpha/pi)^(3/2) exp(-alpha r^2), sampled with
+    weight D^3, is erf(sqrt(alpha) r)/r (2 sqrt(alpha/pi) at r = 0).  A lost
+    2 pi, a wrong sign of TR or a wrong far-field scale fails by far more."""
+    from scipy.special import erf
+    for n, d, al in [(21, 0.4, 1.0), (31, 0.3, 1.0)]:
+        kern = orc.coulomb_kernel(n, d)
+        _, _, pts = orc.sinedvr(n, d)
+        X = np.meshgrid(pts, pts, pts, indexing="ij")
+        c = (0.13, -0.21, 0.05)
+        r2 = sum((x - cc) ** 2 for x, cc in zip(X, c))
+        rho = (al / np.pi) ** 1.5 * np.exp(-al * r2) * d ** 3
+        V = orc.op_tinv(n, kern, rho.ravel(order="F"))[:, 0].reshape((n, n, n), order="F")
+        r = np.sqrt(r2)
+        exact = erf(np.sqrt(al) * r) / r
+        assert relerr(V, exact) < 1e-6                   # observed 3e-7 / 4e-7
+
+
 # ------------------------------------------------------- a5/a6/a7 op_tinv
 @pytest.mark.parametrize("n", [3, 5, 6])
 def test_op_tinv_brute_force(orc, n):
@@ -264,6 +283,91 @@ def test_ecs_functions(orc):
         assert abs((Dp - Dm) / (2 * h) - DD0) < 1e-7
     ij, sd = orc.ecs_arrays(5, 0.4, th, st, 10.0, 1.0)                    # ramp outside the grid
     assert np.all(ij == 1.0) and not sd.any()
+
+
+def _ecs_axis(orc, pts, ecs):
+    v = np.array([orc.ecs_point(p, *ecs) for p in pts])
+    return v[:, 1], v[:, 2], v[:, 3]                                       # J, J', J''
+
+
+@pytest.mark.parametrize("ecs", [(0.3, 1.0, 3.75, 2.25), (0.5, 1.3, 2.0, 3.0)])
+def test_ecs_scalediag_pointwise(orc, ecs):
+    """scalediag (SD/init_sinc_new.f90:517-521) is the diagonal remainder that
+    turns J^-1 (-1/2 d^2) J^-1 into the symmetric complex-scaled kinetic
+    operator -1/2 A d(A^2 d(A f)), A = J^(-1/2):
+        s(x) = -1/2 [A (A^2 A')' - A^2 (A^2)''].
+    Evaluated here by finite differences of J alone (no J', J'' from the
+    oracle), at the grid points of one axis with the other two at the centre
+    (their terms vanish: |x| <= R0)."""
+    n, d = 41, 0.3
+    _, _, pts = orc.sinedvr(n, d)
+    ij, sd = orc.ecs_arrays(n, d, *ecs)
+    c = n // 2
+    assert abs(pts[c]) < 1e-12
+    h = 1e-3
+    w1 = np.array([1, -8, 0, 8, -1]) / (12 * h)                          # 4th-order first derivative
+    w2 = np.array([-1, 16, -30, 16, -1]) / (12 * h * h)                  # 4th-order second derivative
+    off = np.arange(-2, 3) * h
+
+    def A(x):
+        return 1.0 / np.sqrt(orc.ecs_point(x, *ecs)[1])
+
+    def dA(x):
+        return w1 @ np.array([A(x + o) for o in off])
+
+    R0, S = ecs[2], ecs[3]
+    checked = 0
+    for i, x in enumerate(pts):
+        got = sd[i + n * c + n * n * c]
+        if abs(abs(x) - R0) < 0.05 or abs(abs(x) - R0 - S) < 0.05:
+            continue                                                       # stencil would straddle a ramp end
+        u = w1 @ np.array([A(x + o) ** 2 * dA(x + o) for o in off])       # (A^2 A')'
+        v = w2 @ np.array([A(x + o) ** 2 for o in off])                    # (A^2)''
+        ref = -0.5 * (A(x) * u - A(x) ** 2 * v)
+        assert abs(got - ref) < 1e-7, (x, got, ref)
+        checked += R0 < abs(x) < R0 + S
+    assert checked >= 6                                                    # points inside the ramps
+
+
+@pytest.mark.parametrize("ecs", [(0.3, 1.0, 2.0, 4.0), (0.5, 1.3, 2.0, 4.0)])
+def test_ecs_mult_ke_closed_form(orc, ecs):
+    """ECS mult_ke (SD/coreproject.f90:1341-1388) on a smooth, well-resolved
+    complex Gaussian whose support lies inside the ramps of two axes, against
+    the closed form of the complex-scaled kinetic operator,
+        sum_d -1/2 [J^-2 f'' - 2 J^-3 J' f' + (5/4 J^-4 J'^2 - 1/2 J^-3 J'') f],
+    with f', f'' analytic and J, J', J'' from the ECS functions (pinned by
+    finite differences in test_ecs_functions).  The same comparison rejects
+    scalediag tables with the coefficients swapped, the sign flipped or a
+    power of J changed (errors 2e-4 .. 7e-3 against < 1e-7 here)."""
+    n, d = 81, 0.15
+    ke, _, pts = orc.sinedvr(n, d)
+    ij, sd = orc.ecs_arrays(n, d, *ecs)
+    X = np.meshgrid(pts, pts, pts, indexing="ij")
+    c, al, kv = (4.0, -3.6, 0.3), 4.0, (0.8, -0.5, 0.3)
+    f = np.exp(1j * sum(k * x for k, x in zip(kv, X)) - al * sum((x - cc) ** 2 for x, cc in zip(X, c)))
+    J, D, DD = _ecs_axis(orc, pts, ecs)
+    exact = np.zeros_like(f)
+    for ax in range(3):
+        g = 1j * kv[ax] - 2 * al * (X[ax] - c[ax])
+        sh = [1, 1, 1]
+        sh[ax] = n
+        j, dj, ddj = (a.reshape(sh) for a in (J, D, DD))
+        exact += -0.5 * ((g * g - 2 * al) * f / j ** 2 - 2 * dj / j ** 3 * g * f
+                         + (1.25 * dj ** 2 / j ** 4 - 0.5 * ddj / j ** 3) * f)
+    out = orc.mult_ke(n, ke, f.ravel(order="F"), ij, sd)[:, 0].reshape((n, n, n), order="F")
+    assert relerr(out, exact) < 1e-6                                       # observed 3e-8
+
+    I3 = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+
+    def full(s1):
+        return (s1[I3[0]] + s1[I3[1]] + s1[I3[2]]).ravel(order="F")
+
+    good = 3 / 8 * D ** 2 / J ** 4 - 1 / 4 * DD / J ** 3
+    assert np.abs(full(good) - sd).max() < 1e-15
+    for bad in (1 / 4 * D ** 2 / J ** 4 - 3 / 8 * DD / J ** 3, -good,
+                3 / 8 * D ** 2 / J ** 4 - 1 / 4 * DD / J ** 2, 3 / 8 * D ** 2 / J ** 3 - 1 / 4 * DD / J ** 3):
+        o = orc.mult_ke(n, ke, f.ravel(order="F"), ij, full(bad))[:, 0].reshape((n, n, n), order="F")
+        assert relerr(o, exact) > 1e-5
 
 
 # ------------------------------------------------------ a12/a13 mean field
