@@ -195,8 +195,10 @@ int sdv_actreduced00_dev(sdv_ctx* ctx, const void* inspfs, const void* projspfs,
  * m >= numpoints gives the same linear convolution; sdv_circulant_sizes lists
  * the supported half-edges m (returns their count, writes at most cap).
  * sdv_set_circulant rebuilds the kernel spectrum for half-edge m (0 = the
- * smallest supported m >= numpoints); batches built on the context before the
- * change must be rebuilt.  sdv_set_pair_batch sets how many pair potentials
+ * smallest supported m >= numpoints); a batch built on the context before the
+ * change keeps the spectrum it was built with (it holds its own reference and
+ * its own pair lists, so the context may also be freed while the batch
+ * lives).  sdv_set_pair_batch sets how many pair potentials
  * one launch of the convolution pipeline carries (work buffer = pairs x 2m x
  * numpoints^2 complex); 0 (the default) = every pair of the call, up to an
  * 8 GiB work buffer. */
@@ -249,12 +251,16 @@ int sdv_batch_profile_read(sdv_batch* batch, double* ms, int* count);
  * numspf, numspf) of one orbital set (as call_twoe_matel00's pair-symmetric
  * path), SDV_REQ_KE = mult_ke of its numspf orbitals; inspfs is a device
  * array (totpoints, numspf) that must stay unchanged until the request is
- * done.  Outputs live in one device arena of hbm_budget bytes: a queued
+ * done.  hbm_budget is the server's whole device footprint: two tick work
+ * buffers of tick_bytes and, in the rest, one arena for the outputs (the
+ * inputs and the kernel spectra belong to the caller's contexts); a queued
  * request is admitted (first in, first out) when its output fits, and its
  * block returns to the arena at sdv_server_release.  Admitted pair work is
  * cut into ticks of tick_bytes of convolution work buffer; one tick mixes
  * pairs of several requests in one grouped launch (continuous batching), two
- * ticks are in flight.  precision 1 = FP32 butterflies, 0 = FP64.
+ * ticks are in flight on two streams (the device starts the next while the
+ * last workgroups of the previous drain).  precision 1 = FP32 butterflies,
+ * 0 = FP64.
  * sdv_server_poll retires finished ticks, admits and launches (wait = 1:
  * until every admitted and admissible request is done); sdv_server_result
  * gives the state (0 queued, 1 admitted, 2 done), the device output when done
@@ -269,7 +275,10 @@ int sdv_server_submit(sdv_server* server, sdv_ctx* ctx, int kind, const void* in
 int sdv_server_poll(sdv_server* server, int wait, int* ncompleted);
 int sdv_server_result(sdv_server* server, long long request_id, void** out, int* state, double* latency_ms,
                       double* queue_ms);
-/* copy a done request's output (out_bytes of its kind) to device memory dst */
+/* copy a done request's output (out_bytes of its kind) to device memory dst,
+ * enqueued on `stream`; a release right after it is safe: the output's block
+ * returns to the arena only once the copy has finished.  Other readers of the
+ * pointer that sdv_server_result returns must finish before the release. */
 int sdv_server_copy_result(sdv_server* server, long long request_id, void* dst, void* stream);
 int sdv_server_release(sdv_server* server, long long request_id);
 int sdv_server_stats(sdv_server* server, long long* queued, long long* in_flight, long long* arena_used,

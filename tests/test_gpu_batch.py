@@ -146,3 +146,36 @@ def test_batch_rejects_mismatched_numspf(torch_dev):
             Batch([(s, bad)])
     finally:
         s.numspf -= 1
+
+
+def test_batch_outlives_context_changes(torch_dev):
+    """A batch keeps its own reference on each set's kernel spectrum and its
+    own pair lists: changing a context's circulant size, or freeing the
+    context, leaves the batch computing (and rebuilding its schedule) with
+    the spectrum it was built with, bit for bit."""
+    torch = torch_dev
+    from sincdvr import Batch
+    sets = make_sets(torch, SETS[3:6])
+    b = Batch([(s, p) for s, p, _ in sets], precision=0)
+    base = [o.clone() for o in b.run()]
+    torch.cuda.synchronize()
+    s0 = sets[0][0]
+    say("context circulant change under a batch")
+    s0.set_circulant(64)                    # n = 21: default half-edge 21, now 64
+    for o in b.outputs:
+        o.zero_()
+    outs = b.run()
+    torch.cuda.synchronize()
+    for o, r in zip(outs, base):
+        assert torch.equal(o, r)
+    say("context freed under a batch, then schedule rebuilt")
+    s0.close()
+    sets[1][0].close()
+    b.set_schedule("fused", 1, 0)
+    for o in b.outputs:
+        o.zero_()
+    outs = b.run()
+    torch.cuda.synchronize()
+    for o, r in zip(outs, base):
+        assert torch.equal(o, r)
+    b.close()

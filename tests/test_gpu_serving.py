@@ -97,7 +97,9 @@ def test_admission_waits_for_memory(torch_dev, world):
     from sincdvr.serving import COULOMB, DONE, QUEUED, Server
     w = world[0]
     out_bytes = w["s"].totpoints * w["s"].numspf ** 2 * 16
-    srv = Server(int(out_bytes * 1.5), 1 << 26, precision=1)   # room for one output
+    tick = 1 << 26
+    work = 2 * (tick + 256)                                     # the budget holds the two tick buffers
+    srv = Server(work + int(out_bytes * 1.5), tick, precision=1)   # room for one output
     ids = [srv.submit(w["s"], COULOMB, w["t"]) for _ in range(3)]
     srv.poll(wait=True)
     states = [srv.status(i)[0] for i in ids]
@@ -119,7 +121,9 @@ def test_bad_requests_refused(torch_dev, world):
     from sincdvr import SdvError
     from sincdvr.serving import COULOMB, Server
     w = world[0]
-    srv = Server(1 << 20, 1 << 26, precision=1)                # output does not fit the budget
+    with pytest.raises(SdvError):                              # budget below the two tick buffers
+        Server(1 << 20, 1 << 26, precision=1)
+    srv = Server(2 * ((1 << 26) + 256) + (1 << 20), 1 << 26, precision=1)   # output does not fit the rest
     with pytest.raises(SdvError):
         srv.submit(w["s"], COULOMB, w["t"])
     srv.close()
@@ -140,4 +144,36 @@ def test_poisson_stream_small(torch_dev):
     say("latency", serving.percentiles(lat[serving.COULOMB]), "wall", wall)
     assert len(lat[serving.COULOMB]) + len(lat[serving.KE]) == 30
     assert srv.stats()["arena_used"] == 0
+    srv.close()
+
+
+def test_release_right_after_copy(torch_dev, world):
+    """sdv_server_copy_result enqueues the copy on the caller's stream; a
+    release right after it must not hand the output's block to the next
+    request before the copy has read it.  The copy stream is held busy so the
+    next request's kernels run first."""
+    torch = torch_dev
+    from sincdvr.serving import COULOMB, Server
+    w0, w3 = world[0], world[3]                       # same grid and block size class
+    srv = Server(2 * ((1 << 26) + 256) + w0["s"].totpoints * 16 * 16 + 4096, 1 << 26, precision=1)
+    rid = srv.submit(w0["s"], COULOMB, w0["t"])
+    srv.poll(wait=True)
+    say("copy on a busy stream, then release and reuse the block")
+    if hasattr(torch.cuda, "_sleep"):                # the copy waits behind this on the current stream
+        torch.cuda._sleep(200_000_000)
+    else:
+        a = torch.randn(8192, 8192, device="cuda")
+        for _ in range(8):
+            a = a @ a * 1e-3
+    got = srv.result(rid)
+    srv.release(rid)
+    rid2 = srv.submit(w3["s"], COULOMB, w3["t"])     # needs the same arena block
+    srv.poll(wait=True)
+    assert srv.status(rid2)[0] == 2
+    torch.cuda.synchronize()
+    w0["s"].set_precision(1)
+    ref = w0["s"].twoereduced(1, w0["s"].numspf, w0["t"])
+    torch.cuda.synchronize()
+    assert torch.equal(got, ref)
+    srv.release(rid2)
     srv.close()

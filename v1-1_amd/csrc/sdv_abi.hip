@@ -198,6 +198,15 @@ int sdv::spectrum_acquire(int n, double spacing, int nbig, int nsmall, int min_m
   return SDV_OK;
 }
 
+void sdv::spectrum_retain(Spectrum* s) {
+  std::lock_guard<std::mutex> lk(g_spec_mu);
+  for (auto& kv : g_specs)
+    if (kv.second.s == s) {
+      kv.second.refs++;
+      return;
+    }
+}
+
 void sdv::spectrum_release(Spectrum* s) {
   if (!s) return;
   std::lock_guard<std::mutex> lk(g_spec_mu);
@@ -793,7 +802,12 @@ static int prof_read(const Prof& p, double* ms, int* count) {
 }
 
 struct sdv_batch {
-  std::vector<sdv_ctx*> ctxs;
+  // The batch holds its own reference on every set's kernel spectrum and its
+  // own copy of every set's pair list, so a context may change its circulant
+  // (sdv_set_circulant) or be freed while the batch lives: the batch keeps
+  // computing with the spectrum it was built with.
+  std::vector<Spectrum*> specs;
+  std::vector<std::vector<PairItem>> items;  // reserved up front: addresses stay put
   std::vector<BatchSetHost> sets;
   long long pairs = 0;  // pair potentials delivered per step (unordered pairs of every set)
   BatchPlan plan;
@@ -891,32 +905,35 @@ int sdv_batch_create(int nsets, const sdv_batch_set* sets, int precision, sdv_ba
   SDV_REQUIRE(precision == 0 || precision == 1, SDV_ERR_ARG, "precision must be 0 or 1");
   sdv_batch* b = new sdv_batch();
   b->precision = precision;
+  b->items.reserve(nsets);
   for (int i = 0; i < nsets; ++i) {
     sdv_ctx* c = sets[i].ctx;
     if (!c || !sets[i].inspfs || !sets[i].twoereduced || sets[i].numspf != c->N) {
       set_error("batch set " + std::to_string(i) + ": null pointer or numspf differs from its context");
-      delete b;
+      sdv_batch_free(b);
       return SDV_ERR_ARG;
     }
     IdxPlan* pl = nullptr;
     int rc = twoe_plan(c, 1, c->N, true, &pl);
     if (rc != SDV_OK) {
-      delete b;
+      sdv_batch_free(b);
       return rc;
     }
+    spectrum_retain(c->spec);
+    b->specs.push_back(c->spec);
+    b->items.push_back(pl->items);
     BatchSetHost h;
     h.spec = c->spec;
-    h.items = &pl->items;
+    h.items = &b->items.back();
     b->pairs += (long long)pl->cols.size();
     h.phi = sets[i].inspfs;
     h.out = sets[i].twoereduced;
     h.cnorm = c->cfg.cnormflag;
     b->sets.push_back(h);
-    b->ctxs.push_back(c);
   }
   int rc = batch_build(b);
   if (rc != SDV_OK) {
-    delete b;
+    sdv_batch_free(b);
     return rc;
   }
   *out = b;
@@ -979,6 +996,7 @@ int sdv_batch_free(sdv_batch* b) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
+  for (Spectrum* sp : b->specs) spectrum_release(sp);
   delete b;
   return SDV_OK;
 }

@@ -80,6 +80,7 @@ int twoe_plan(sdv_ctx* c, int low, int high, bool sym, IdxPlan** out);
 // spacing, tooth parameters, circulant half-edge): acquired at sdv_init /
 // sdv_set_circulant, released at sdv_free; built once, kept resident.
 int spectrum_acquire(int n, double spacing, int nbig, int nsmall, int min_m, hipStream_t st, Spectrum** out);
+void spectrum_retain(Spectrum* s);   // one more reference on a cached spectrum (batches hold one per set)
 void spectrum_release(Spectrum* s);
 
 }  // namespace sdv

@@ -11,9 +11,11 @@
 //     several requests (and requests of different grids) in one grouped batch
 //     launch, so requests that arrive while others run join the next tick and
 //     their work overlaps on the device;
-//   * two ticks in flight (double-buffered work buffers and index tables):
-//     the host builds tick t+1 while the device runs tick t;
-//   * a request is done when the tick holding its last pairs has completed.
+//   * two ticks in flight, each on its own stream with its own work buffer
+//     and index tables: the host builds tick t+1 while the device runs tick
+//     t, and the device starts tick t+1 while tick t's last workgroups drain;
+//   * a request is done when every tick holding its work has completed;
+//   * the HBM budget covers the output arena and the two tick work buffers.
 // Each pair is computed by the same kernels whatever else is in the tick, so
 // a request's result does not depend on the mix.
 #include <algorithm>
@@ -92,13 +94,18 @@ struct Req {
   size_t out_off = 0, out_bytes = 0;
   const std::vector<PairItem>* items = nullptr;  // Coulomb: pair list of the context (1..N, symmetric)
   size_t next = 0;                                // next pair to schedule
-  long long last_tick = -1;                       // tick holding this request's last work
+  int inflight = 0;                               // launched ticks holding its work, not yet retired
+  hipEvent_t copy_ev = nullptr;                   // after the last sdv_server_copy_result of the output
   int state = ST_QUEUED;
   double t_submit = 0, t_admit = 0, t_done = 0;
 };
 
-// Device-side and pinned host-side tables of one tick slot.
+// One tick slot: its stream (plus the forked streams of the grouped launch),
+// work buffer, device tables and pinned staging.
 struct Slot {
+  hipStream_t st = nullptr;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> fork_ev;
   void* X = nullptr;  // work buffer (tick_bytes)
   void* dtab = nullptr;
   size_t dtab_cap = 0;
@@ -106,8 +113,7 @@ struct Slot {
   size_t htab_cap = 0;
   hipEvent_t done = nullptr;
   long long tick = -1;  // tick currently using the slot (-1: free)
-  std::vector<std::vector<PairItem>> slices;
-  std::vector<long long> finishing;  // requests whose last work is in this tick
+  std::vector<long long> members;  // requests with work in this tick
 };
 
 }  // namespace
@@ -117,15 +123,21 @@ struct sdv_server {
   size_t budget = 0, tick_bytes = 0;
   void* arena_mem = nullptr;
   Arena arena;
-  hipStream_t stream = nullptr;
-  std::vector<hipStream_t> side;
-  std::vector<hipEvent_t> fork_ev;
   Slot slots[2];
   long long next_id = 1, next_tick = 0;
   std::map<long long, Req> reqs;
   std::deque<long long> queue;    // submitted, not admitted (FIFO)
   std::deque<long long> active;   // admitted Coulomb requests with pairs left to schedule
   std::deque<long long> ke_ready; // admitted kinetic requests not yet launched
+  // Released outputs whose copies (sdv_server_copy_result, on the caller's
+  // stream) may still be reading them: their blocks return to the arena once
+  // the copy's event has completed, so a later request cannot overwrite them.
+  struct Deferred {
+    size_t off, bytes;
+    hipEvent_t ev;
+  };
+  std::vector<Deferred> deferred;
+  std::vector<hipEvent_t> ev_pool;
   int ncu = 256;
   long long completed = 0;
 };
@@ -157,28 +169,48 @@ static int grow_slot_tables(Slot& s, size_t bytes) {
   return SDV_OK;
 }
 
-// Mark requests of completed ticks done.
+// Retire completed ticks; a request is done once every tick holding its
+// work has completed and none of its work is left to schedule.
 static void retire(sdv_server* v, bool wait_all) {
   for (Slot& s : v->slots) {
     if (s.tick < 0) continue;
     if (!wait_all && hipEventQuery(s.done) != hipSuccess) continue;
     if (wait_all) (void)hipEventSynchronize(s.done);
     const double t = now_ms();
-    for (long long id : s.finishing) {
+    for (long long id : s.members) {
       auto it = v->reqs.find(id);
-      if (it != v->reqs.end() && it->second.last_tick == s.tick && it->second.state == ST_ADMITTED &&
-          (it->second.kind != SDV_REQ_COULOMB || it->second.next >= it->second.items->size())) {
-        it->second.state = ST_DONE;
-        it->second.t_done = t;
+      if (it == v->reqs.end()) continue;
+      Req& r = it->second;
+      r.inflight--;
+      if (r.inflight == 0 && r.state == ST_ADMITTED && (r.kind != SDV_REQ_COULOMB || r.next >= r.items->size())) {
+        r.state = ST_DONE;
+        r.t_done = t;
         v->completed++;
       }
     }
-    s.finishing.clear();
+    s.members.clear();
     s.tick = -1;
   }
 }
 
+// Return to the arena the released blocks whose copies have finished (wait:
+// block until they have).
+static void reclaim(sdv_server* v, bool wait) {
+  size_t k = 0;
+  for (auto& d : v->deferred) {
+    if (wait) (void)hipEventSynchronize(d.ev);
+    if (wait || hipEventQuery(d.ev) == hipSuccess) {
+      v->arena.release(d.off, d.bytes);
+      v->ev_pool.push_back(d.ev);
+    } else {
+      v->deferred[k++] = d;
+    }
+  }
+  v->deferred.resize(k);
+}
+
 static void admit(sdv_server* v) {
+  reclaim(v, false);
   while (!v->queue.empty()) {
     Req& r = v->reqs[v->queue.front()];
     size_t off = 0;
@@ -195,7 +227,9 @@ static void admit(sdv_server* v) {
 }
 
 // Build and launch one tick in a free slot; *launched = 0 when there was
-// nothing to do or no free slot.
+// nothing to do or no free slot.  Request and slot state change only after
+// every launch of the tick has been enqueued: on an error nothing is marked,
+// and the same work is scheduled again by the next poll.
 static int launch_tick(sdv_server* v, int* launched) {
   *launched = 0;
   if (v->active.empty() && v->ke_ready.empty()) return SDV_OK;
@@ -206,11 +240,33 @@ static int launch_tick(sdv_server* v, int* launched) {
       break;
     }
   if (!slot) return SDV_OK;
-  const long long tick = v->next_tick++;
-  slot->tick = tick;
-  slot->finishing.clear();
-  slot->slices.clear();
   char* base = (char*)v->arena_mem;
+
+  // Coulomb pairs, FIFO over admitted requests, up to the tick's work buffer
+  // (planned without touching the requests)
+  std::vector<std::vector<PairItem>> slices;
+  slices.reserve(v->active.size());  // no reallocation: the sets keep pointers to the slices
+  std::vector<std::pair<long long, size_t>> takes;  // (request, pairs taken)
+  std::vector<BatchSetHost> sets;
+  size_t used = 0;
+  for (long long id : v->active) {
+    Req& r = v->reqs[id];
+    const size_t per = pair_work_bytes(r.ctx->spec->g, v->precision);
+    const size_t left = r.items->size() - r.next;
+    const size_t take = std::min(left, (v->tick_bytes - used) / per);
+    if (take == 0) break;  // tick full (a single pair always fits: checked at submit)
+    slices.emplace_back(r.items->begin() + r.next, r.items->begin() + r.next + take);
+    takes.emplace_back(id, take);
+    used += take * per;
+    BatchSetHost h;
+    h.spec = r.ctx->spec;
+    h.items = &slices.back();
+    h.phi = r.in;
+    h.out = base + r.out_off;
+    h.cnorm = r.ctx->cfg.cnormflag;
+    sets.push_back(h);
+    if (take < left) break;  // the tick is full
+  }
 
   // kinetic requests: all that are ready
   for (long long id : v->ke_ready) {
@@ -225,43 +281,7 @@ static int launch_tick(sdv_server* v, int* launched) {
     a.mask = c->mask;
     a.in = r.in;
     a.out = base + r.out_off;
-    SDV_TRY(ke_launch(a, v->precision, v->stream, nullptr));
-    r.last_tick = tick;
-    slot->finishing.push_back(id);
-  }
-  v->ke_ready.clear();
-
-  // Coulomb pairs, FIFO over admitted requests, up to the tick's work buffer
-  std::vector<BatchSetHost> sets;
-  size_t used = 0;
-  slot->slices.reserve(v->active.size());  // no reallocation: the sets keep pointers to the slices
-  while (!v->active.empty()) {
-    Req& r = v->reqs[v->active.front()];
-    const Geom& g = r.ctx->spec->g;
-    const size_t per = pair_work_bytes(g, v->precision);
-    size_t take = 0;
-    const size_t left = r.items->size() - r.next;
-    while (take < left && used + per <= v->tick_bytes) {
-      used += per;
-      ++take;
-    }
-    if (take == 0) break;  // tick full (a single pair always fits: checked at submit)
-    slot->slices.emplace_back(r.items->begin() + r.next, r.items->begin() + r.next + take);
-    r.next += take;
-    BatchSetHost h;
-    h.spec = r.ctx->spec;
-    h.items = &slot->slices.back();
-    h.phi = r.in;
-    h.out = base + r.out_off;
-    h.cnorm = r.ctx->cfg.cnormflag;
-    sets.push_back(h);
-    r.last_tick = tick;
-    if (r.next >= r.items->size()) {
-      slot->finishing.push_back(r.id);
-      v->active.pop_front();
-    } else {
-      break;  // the tick is full
-    }
+    SDV_TRY(ke_launch(a, v->precision, slot->st, nullptr));
   }
   if (!sets.empty()) {
     BatchPlan bp = batch_plan(sets, v->precision, BATCH_GROUPED, 1, v->ncu);
@@ -279,7 +299,7 @@ static int launch_tick(sdv_server* v, int* launched) {
     std::memcpy(slot->htab + o_items, bp.items.data(), bp.items.size() * batch_item_bytes());
     std::memcpy(slot->htab + o_wx, bp.wx.data(), bp.wx.size() * batch_work_bytes());
     std::memcpy(slot->htab + o_wy, bp.wy.data(), bp.wy.size() * batch_work_bytes());
-    SDV_HIP_CHECK(hipMemcpyAsync(slot->dtab, slot->htab, total, hipMemcpyHostToDevice, v->stream));
+    SDV_HIP_CHECK(hipMemcpyAsync(slot->dtab, slot->htab, total, hipMemcpyHostToDevice, slot->st));
     BatchDev d;
     char* dt = (char*)slot->dtab;
     d.sets = dt + o_sets;
@@ -287,28 +307,56 @@ static int launch_tick(sdv_server* v, int* launched) {
     d.wx = (const BWork*)(dt + o_wx);
     d.wy = (const BWork*)(dt + o_wy);
     d.X = slot->X;
-    d.nside = (int)v->side.size();
-    d.side = d.nside ? v->side.data() : nullptr;
-    d.ev = d.nside ? v->fork_ev.data() : nullptr;
-    SDV_TRY(batch_launch(bp, d, v->stream, nullptr));
+    d.nside = (int)slot->side.size();
+    d.side = d.nside ? slot->side.data() : nullptr;
+    d.ev = d.nside ? slot->fork_ev.data() : nullptr;
+    SDV_TRY(batch_launch(bp, d, slot->st, nullptr));
   }
-  SDV_HIP_CHECK(hipEventRecord(slot->done, v->stream));
+  SDV_HIP_CHECK(hipEventRecord(slot->done, slot->st));
+
+  // commit
+  slot->tick = v->next_tick++;
+  slot->members.clear();
+  for (long long id : v->ke_ready) {
+    v->reqs[id].inflight++;
+    slot->members.push_back(id);
+  }
+  v->ke_ready.clear();
+  for (const auto& tk : takes) {
+    Req& r = v->reqs[tk.first];
+    r.next += tk.second;
+    r.inflight++;
+    slot->members.push_back(r.id);
+    if (r.next >= r.items->size()) {
+      // only the front request can be finished here: takes run over the FIFO in order
+      v->active.pop_front();
+    }
+  }
   *launched = 1;
   return SDV_OK;
 }
 
 static void server_release(sdv_server* v) {
-  if (v->stream) (void)hipStreamSynchronize(v->stream);
+  for (Slot& s : v->slots)
+    if (s.st) (void)hipStreamSynchronize(s.st);
   for (Slot& s : v->slots) {
     (void)hipFree(s.X);
     (void)hipFree(s.dtab);
     if (s.htab) (void)hipHostFree(s.htab);
     if (s.done) (void)hipEventDestroy(s.done);
+    for (hipStream_t st : s.side) (void)hipStreamDestroy(st);
+    for (hipEvent_t e : s.fork_ev) (void)hipEventDestroy(e);
+    if (s.st) (void)hipStreamDestroy(s.st);
   }
-  for (hipStream_t st : v->side) (void)hipStreamDestroy(st);
-  for (hipEvent_t e : v->fork_ev) (void)hipEventDestroy(e);
+  for (auto& d : v->deferred) (void)hipEventSynchronize(d.ev);
+  for (auto& d : v->deferred) (void)hipEventDestroy(d.ev);
+  for (hipEvent_t e : v->ev_pool) (void)hipEventDestroy(e);
+  for (auto& kv : v->reqs)
+    if (kv.second.copy_ev) {
+      (void)hipEventSynchronize(kv.second.copy_ev);
+      (void)hipEventDestroy(kv.second.copy_ev);
+    }
   (void)hipFree(v->arena_mem);
-  if (v->stream) (void)hipStreamDestroy(v->stream);
 }
 
 extern "C" {
@@ -320,37 +368,44 @@ int sdv_server_create(long long hbm_budget, long long tick_bytes, int precision,
   v->precision = precision;
   v->budget = (size_t)hbm_budget;
   v->tick_bytes = (size_t)tick_bytes;
+  // the budget holds the two tick work buffers; the rest is the output arena
+  const size_t work = 2 * align_up(v->tick_bytes + 256, 256);
+  if (v->budget <= work) {
+    delete v;
+    set_error("HBM budget does not exceed the two tick work buffers");
+    return SDV_ERR_ARG;
+  }
+  const size_t arena_bytes = v->budget - work;
   int dev = 0;
   int rc = SDV_OK;
   if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&v->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) != hipSuccess) {
+      hipDeviceGetAttribute(&v->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
     set_error("HIP set-up of the server failed");
     rc = SDV_ERR_HIP;
   }
-  for (int i = 0; i < 4 && rc == SDV_OK; ++i) {
-    hipStream_t st = nullptr;
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = SDV_ERR_HIP;
-    else v->side.push_back(st);
-  }
-  for (int i = 0; i < 5 && rc == SDV_OK; ++i) {
-    hipEvent_t e = nullptr;
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) rc = SDV_ERR_HIP;
-    else v->fork_ev.push_back(e);
-  }
   for (Slot& s : v->slots) {
-    if (rc != SDV_OK) break;
-    if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) rc = SDV_ERR_HIP;
-    else if (hipMalloc(&s.X, v->tick_bytes + 256) != hipSuccess) rc = SDV_ERR_NOMEM;
+    if (rc == SDV_OK && hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess) rc = SDV_ERR_HIP;
+    for (int i = 0; i < 4 && rc == SDV_OK; ++i) {
+      hipStream_t st = nullptr;
+      if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = SDV_ERR_HIP;
+      else s.side.push_back(st);
+    }
+    for (int i = 0; i < 5 && rc == SDV_OK; ++i) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) rc = SDV_ERR_HIP;
+      else s.fork_ev.push_back(e);
+    }
+    if (rc == SDV_OK && hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) rc = SDV_ERR_HIP;
+    if (rc == SDV_OK && hipMalloc(&s.X, v->tick_bytes + 256) != hipSuccess) rc = SDV_ERR_NOMEM;
   }
-  if (rc == SDV_OK && hipMalloc(&v->arena_mem, v->budget) != hipSuccess) rc = SDV_ERR_NOMEM;
+  if (rc == SDV_OK && hipMalloc(&v->arena_mem, arena_bytes) != hipSuccess) rc = SDV_ERR_NOMEM;
   if (rc != SDV_OK) {
     if (rc == SDV_ERR_NOMEM) set_error("device allocation of the server arena or work buffers failed");
     server_release(v);
     delete v;
     return rc;
   }
-  v->arena.init(v->budget);
+  v->arena.init(arena_bytes);
   *out = v;
   return SDV_OK;
 }
@@ -374,7 +429,8 @@ int sdv_server_submit(sdv_server* v, sdv_ctx* c, int kind, const void* inspfs, l
   } else {
     r.out_bytes = col * c->N;
   }
-  SDV_REQUIRE(align_up(r.out_bytes, 256) <= v->budget, SDV_ERR_ARG, "request output exceeds the HBM budget");
+  SDV_REQUIRE(align_up(r.out_bytes, 256) <= v->arena.size, SDV_ERR_ARG,
+              "request output exceeds the HBM budget left beside the tick work buffers");
   r.t_submit = now_ms();
   v->reqs[r.id] = r;
   v->queue.push_back(r.id);
@@ -401,8 +457,10 @@ int sdv_server_poll(sdv_server* v, int wait, int* ncompleted) {
       int launched = 1;
       while (launched) SDV_TRY(launch_tick(v, &launched));
       if (!v->queue.empty() && v->active.empty() && v->ke_ready.empty() && v->slots[0].tick < 0 &&
-          v->slots[1].tick < 0)
-        break;  // queued requests wait for released outputs
+          v->slots[1].tick < 0) {
+        if (v->deferred.empty()) break;  // queued requests wait for outputs still to be released
+        reclaim(v, true);                // or for copies of released outputs to finish
+      }
     }
   }
   if (ncompleted) *ncompleted = (int)(v->completed - before);
@@ -426,8 +484,18 @@ int sdv_server_copy_result(sdv_server* v, long long id, void* dst, void* stream)
   auto it = v->reqs.find(id);
   SDV_REQUIRE(it != v->reqs.end(), SDV_ERR_ARG, "unknown or released request");
   SDV_REQUIRE(it->second.state == ST_DONE, SDV_ERR_ARG, "request not done");
-  SDV_HIP_CHECK(hipMemcpyAsync(dst, (char*)v->arena_mem + it->second.out_off, it->second.out_bytes,
-                               hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  Req& r = it->second;
+  SDV_HIP_CHECK(hipMemcpyAsync(dst, (char*)v->arena_mem + r.out_off, r.out_bytes, hipMemcpyDeviceToDevice,
+                               (hipStream_t)stream));
+  if (!r.copy_ev) {
+    if (!v->ev_pool.empty()) {
+      r.copy_ev = v->ev_pool.back();
+      v->ev_pool.pop_back();
+    } else {
+      SDV_HIP_CHECK(hipEventCreateWithFlags(&r.copy_ev, hipEventDisableTiming));
+    }
+  }
+  SDV_HIP_CHECK(hipEventRecord(r.copy_ev, (hipStream_t)stream));
   return SDV_OK;
 }
 
@@ -436,7 +504,11 @@ int sdv_server_release(sdv_server* v, long long id) {
   auto it = v->reqs.find(id);
   SDV_REQUIRE(it != v->reqs.end(), SDV_ERR_ARG, "unknown or released request");
   SDV_REQUIRE(it->second.state == ST_DONE, SDV_ERR_ARG, "request not done");
-  v->arena.release(it->second.out_off, it->second.out_bytes);
+  const Req& r = it->second;
+  if (r.copy_ev)
+    v->deferred.push_back(sdv_server::Deferred{r.out_off, r.out_bytes, r.copy_ev});
+  else
+    v->arena.release(r.out_off, r.out_bytes);
   v->reqs.erase(it);
   return SDV_OK;
 }
@@ -444,6 +516,7 @@ int sdv_server_release(sdv_server* v, long long id) {
 int sdv_server_stats(sdv_server* v, long long* queued, long long* in_flight, long long* arena_used,
                      long long* arena_peak, long long* ticks) {
   SDV_REQUIRE(v, SDV_ERR_ARG, "null server");
+  reclaim(v, false);
   long long q = (long long)v->queue.size(), f = 0;
   for (auto& kv : v->reqs)
     if (kv.second.state == ST_ADMITTED) ++f;
