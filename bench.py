@@ -16,13 +16,17 @@ orbitals already resident in HBM.  `value` is the plain FP64 form; beside it:
     p50/p99 latency at 70% of each form's measured capacity;
   * two-electron matrix elements and the kinetic apply, timed separately.
 
-Roofline: algorithmic bytes per convolution follow the pencil model
-B = s (2 n^3 + 4 M n^2 + 4 M^2 n) (s = 16 FP64, 8 FP32), split over the
-kernels by the passes each performs: k_xfwd = x-forward (R n^3, W M n^2),
-k_yz = y-forward + z (fused multiply) + y-inverse (R+W 2 M n^2 + 4 M^2 n),
-k_xinv = x-inverse (R M n^2, W n^3).  The 136 pair potentials take 128
-convolutions: the 16 diagonal pair densities |phi_a|^2 are real and run two
-per complex convolution.
+Roofline of the dominant kernel, the fused y/z kernel k_yz: its algorithmic
+HBM bytes are what the fused kernel must move, read the M n^2 x-forward
+output once and write the M n^2 input of the x-inverse once, s (2 M n^2) per
+convolution (s = 16 FP64, 8 FP32; 34.0 MB FP64 at 81^3), plus the spectrum
+slices once per launch; `traffic` is what it does move (calibrated rocprofv3
+counters, profiles/), and `bound` is what limits it by the SQ counters
+(VALU issue and LDS exchanges, not HBM).  The path's effective bandwidth
+against a streaming pencil implementation, B = s (2 n^3 + 4 M n^2 + 4 M^2 n)
+per convolution, is reported beside it as path_effective_pencil_frac.  The
+136 pair potentials take 128 convolutions: the 16 diagonal pair densities
+|phi_a|^2 are real and run two per complex convolution.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 """
@@ -40,6 +44,18 @@ METRIC = "pair Coulomb potentials/s at 81^3 (mixed vs FP64), % HBM peak, p99 lat
 HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_F64 = 78.6    # TFLOP/s dense, MI355X
 MFMA_PEAK_F32 = 157.3
+
+
+def fused_bytes_per_conv(stage, n, M, s_elem):
+    """Algorithmic HBM bytes of one convolution in each fused kernel: the
+    inputs it must read once and the outputs it must write once."""
+    if stage == "xfwd":        # two orbital columns (n^3 each, FP64) -> M n^2 x-frequency lines
+        return 2 * 16 * n ** 3 + s_elem * M * n * n
+    if stage == "yz":          # M n^2 lines in, M n^2 lines out (the spectrum once per launch: separate)
+        return 2 * s_elem * M * n * n
+    if stage == "xinv":        # M n^2 lines -> one or two output columns (n^3, FP64)
+        return s_elem * M * n * n + 2 * 16 * n ** 3
+    raise ValueError(stage)
 
 
 def alg_bytes_per_pair(stage, n, M, s_elem):
@@ -66,12 +82,26 @@ def time_steps(fn, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(n, dx, phi, cores, nbatch=16):
-    """The oracle (a C port of the reference's CPU path) on `nbatch` spf2b
-    batches of call_twoe_matel00 -- each N densities phi(:,a)*conj(phi(:,b))
-    convolved by 3 x (1-D FFT + transpose) at M = 2n = 162 with the kernel FFT
-    recomputed, as the reference does -- scaled to the full twoereduced (N
-    batches)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, dx, phi, cores, nbatch_ref=1, nbatch_port=2):
+    """The reference's own CPU convolution -- circ3d_sub, myzfft3d and the
+    DFFTPACK backend of myzfft1d compiled from the reference sources into
+    oracle/_ref/libref_conv.so, framed as op_tinv_notscaled -- on `nbatch_ref`
+    spf2b batches of call_twoe_matel00 (each N densities phi(:,a) conj(phi(:,b))
+    at M = 2n = 162 with the kernel FFT recomputed, as the reference does),
+    scaled to the full twoereduced (N batches); beside it the oracle's C
+    restatement of the same algorithm.  Falls back to the restatement alone
+    (kind "port") when oracle/_ref is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
 
@@ -79,18 +109,31 @@ def cpu_baseline(n, dx, phi, cores, nbatch=16):
     oracle.set_threads(cores)
     kern = oracle.coulomb_kernel(n, dx)
     N = phi.shape[1]
-    nbatch = min(nbatch, N)
-    t = 0.0
-    for b in range(nbatch):
-        den = np.asfortranarray(phi * np.conj(phi[:, [b]]))
-        t0 = time.perf_counter()
-        oracle.op_tinv(n, kern, den)
-        t += time.perf_counter() - t0
-    full = t * N / nbatch
-    return {"value": (N * (N + 1) // 2) / full, "unit": "pair Coulomb potentials/s", "cores": cores,
-            "kind": "port",
-            "sample": "%d of the %d spf2b batches of call_twoe_matel00 at 81^3 (each %d densities + the kernel "
-                      "FFT, M=162), %.1f s, scaled to the full twoereduced" % (nbatch, N, N, t)}
+    pairs = N * (N + 1) // 2
+
+    def timed(fn, nb):
+        t = 0.0
+        for b in range(nb):
+            den = np.asfortranarray(phi * np.conj(phi[:, [b]]))
+            t0 = time.perf_counter()
+            fn(den)
+            t += time.perf_counter() - t0
+        return t
+
+    t_port = timed(lambda d: oracle.op_tinv(n, kern, d), nbatch_port)
+    port = {"value": pairs / (t_port * N / nbatch_port), "unit": "pair Coulomb potentials/s", "cores": cores,
+            "sample": "%d of the %d spf2b batches, %.1f s, scaled to the full twoereduced" % (nbatch_port, N, t_port)}
+    host = {"nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "threads_note": "threads = the GPU host's CPU share for one GPU (OMP_NUM_THREADS there)"}
+    if oracle.conv_lib() is None:
+        return dict(port, kind="port", **host)
+    t_ref = timed(lambda d: oracle.ref_op_tinv(n, kern, d, threads=cores), nbatch_ref)
+    return dict({"value": pairs / (t_ref * N / nbatch_ref), "unit": "pair Coulomb potentials/s", "cores": cores,
+                 "kind": "reference",
+                 "sample": "%d of the %d spf2b batches of call_twoe_matel00 at 81^3 (each %d densities + the "
+                           "kernel FFT, M=162, through the reference's circ3d_sub / myzfft3d / DFFTPACK built from "
+                           "its sources), %.1f s, scaled to the full twoereduced" % (nbatch_ref, N, N, t_ref),
+                 "port": port}, **host)
 
 
 def bench_batched_81(torch, s, phi, npairs, steps):
@@ -211,7 +254,10 @@ def main():
     phi_h = inputs.gaussian_orbitals(n, dx, N, cfg["ngauss"], cfg["seed"])
     phi = torch.from_numpy(np.ascontiguousarray(phi_h.T)).cuda()
     print("[bench] building kernel spectrum", file=sys.stderr, flush=True)
-    s = SincDVR(n, dx, N, precision=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s = SincDVR(n, dx, N, precision=0)           # GETINVERSE + kernel placement + spectra, on the device
+    t_setup = time.perf_counter() - t0
     M = s.circulant_edge
     tr = torch.zeros((N, N, n ** 3), dtype=torch.complex128, device="cuda")
     tm = torch.zeros((N, N, N, N), dtype=torch.complex128, device="cuda")
@@ -270,6 +316,18 @@ def main():
         (ke_out["mixed"] - ke_out["fp64"]).abs().max() / ke_out["fp64"].abs().max())
     del ke_out
     tke = ke_res["fp64"]["ms_per_call_16_orbitals"] / 1e3
+    print("[bench] mean field", file=sys.stderr, flush=True)
+    # reducedpot = twoereduced . twoden (S/mean.f90:556-558), twoden a seeded N^2 x N^2 complex matrix
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    twoden = torch.complex(torch.randn((N * N, N * N), dtype=torch.float64, device="cuda", generator=gen),
+                           torch.randn((N * N, N * N), dtype=torch.float64, device="cuda", generator=gen))
+    tmf = time_steps(lambda: s.get_reducedpot(tr, twoden), args.steps, 1, torch)
+    mf_flops = 8.0 * n ** 3 * N ** 4
+    mean_field = {"ms_per_call": tmf * 1e3, "tflops": mf_flops / tmf / 1e12, "peak_tflops_f64": MFMA_PEAK_F64,
+                  "frac": mf_flops / tmf / 1e12 / MFMA_PEAK_F64,
+                  "shape": "(n^3 x N^2) . (N^2 x N^2) complex, 8 n^3 N^4 real flops"}
+    del twoden
 
     # roofline of the dominant kernel of the FP64 form
     stages = ["xfwd", "yz", "xinv"]
@@ -279,9 +337,13 @@ def main():
     nconv = s.convolutions(1, N)                 # 16 diagonal pairs run two per convolution
     pairs_per_launch = npairs * args.steps / launches
     conv_per_launch = nconv * args.steps / launches
-    alg = alg_bytes_per_pair(dom, n, M, 16) * conv_per_launch
+    m = M // 2
+    alg = fused_bytes_per_conv(dom, n, M, 16) * conv_per_launch
+    if dom == "yz":
+        alg += 8.0 * (m + 1) ** 3                 # the FP64 spectrum octant, read once per launch
     achieved = alg / (avg_ms * 1e-3) / 1e9
     traffic = None
+    counters = {}
     tfile = os.path.join(ROOT, "profiles", "traffic_k_yz_81.json")
     if dom == "yz" and os.path.exists(tfile):
         with open(tfile) as f:
@@ -291,6 +353,7 @@ def main():
             traffic = traffic * conv_per_launch / tj["convolutions_per_launch"]
         else:
             traffic = None                            # counters of another work split
+        counters = {k: tj[k] for k in ("valu_busy", "lds_bank_conflict_frac", "waves_per_cu", "source") if k in tj}
     out = {
         "metric": METRIC,
         "value": npairs / t64,
@@ -308,18 +371,30 @@ def main():
                                "(136 unique / 256 ordered pairs), circulant edge %d" % M,
                    "grid": n, "orbitals": N, "unique_pairs": npairs, "circulant_edge": M,
                    "parallelism": "1 GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_" + dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": alg, "pairs_per_launch": pairs_per_launch,
+        "roofline": {"bound": "valu" if dom == "yz" else "hbm", "kernel": "k_" + dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": min(1.0, achieved / HBM_PEAK_GBS),
+                     "traffic": traffic,
+                     "traffic_frac": (traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+                     "alg_bytes_per_launch": alg, "alg_bytes": "fused kernel's floor: read the x-forward "
+                     "output once, write the x-inverse input once (+ spectrum octant)",
+                     "limiter": "VALU issue and LDS exchanges (rocprofv3 SQ counters, profiles/)" if dom == "yz"
+                     else "HBM", "sq_counters": counters or None,
+                     "pairs_per_launch": pairs_per_launch,
                      "convolutions_per_launch": conv_per_launch, "avg_launch_ms": avg_ms},
-        "path_hbm_frac": alg_bytes_per_pair("path", n, M, 16) * nconv / t64 / 1e9 / HBM_PEAK_GBS,
+        "path_effective_pencil_frac": alg_bytes_per_pair("path", n, M, 16) * nconv / t64 / 1e9 / HBM_PEAK_GBS,
+        "path_effective_note": "pencil-model bytes B = s(2n^3 + 4Mn^2 + 4M^2n) per convolution / step time / "
+                               "8 TB/s: the path's speed against a streaming 3-pass pencil FFT at HBM peak; the "
+                               "fused kernels move less than B",
         "stage_ms_per_step_fp64": {k: prof64[k][0] / args.steps for k in stages},
         "mixed": {"value": npairs / t32, "ms_per_step": t32 * 1e3, "speedup_vs_fp64": t64 / t32,
-                  "path_hbm_frac": alg_bytes_per_pair("path", n, M, 8) * nconv / t32 / 1e9 / HBM_PEAK_GBS,
+                  "path_effective_pencil_frac": alg_bytes_per_pair("path", n, M, 8) * nconv / t32 / 1e9
+                  / HBM_PEAK_GBS,
                   "dtype": "f32 butterflies, f64 spectrum construction and reductions",
                   "twoematel_error_vs_fp64": err,
                   "stage_ms_per_step": {k: prof32[k][0] / args.steps for k in stages}},
         "twoematel_ms_fp64": tmat * 1e3,
+        "mean_field_fp64": mean_field,
+        "spectrum_setup_ms": t_setup * 1e3,
         "mult_ke": dict(ke_res, note="per-axis contraction on the matrix cores (FP64 v_mfma_f64_16x16x4f64, "
                                      "mixed v_mfma_f32_16x16x4f32); utilisation = issued matrix flops / peak"),
     }
@@ -335,7 +410,7 @@ def main():
         print("[bench] serving", file=sys.stderr, flush=True)
         out["serving"] = bench_serving(torch, args.quick)
     if not args.no_cpu:
-        cores = min(os.cpu_count() or 1, 16)
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
         print(f"[bench] CPU baseline on {cores} threads", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(n, dx, phi_h, cores)
     else:

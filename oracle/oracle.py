@@ -15,9 +15,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liborc.so")
 REF_SO = os.path.join(HERE, "_ref", "libref_sinc.so")
+CONV_SO = os.path.join(HERE, "_ref", "libref_conv.so")
 
 _lib = None
 _ref = None
+_conv = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -44,6 +46,74 @@ def ref_lib():
     if _ref is None and os.path.exists(REF_SO):
         _ref = ctypes.CDLL(REF_SO)
     return _ref
+
+
+def conv_lib():
+    """The reference's own convolution (circ3d_sub, myzfft3d, DFFTPACK) behind
+    oracle/ref_bind.f90, or None if not built."""
+    global _conv
+    if _conv is None and os.path.exists(CONV_SO):
+        _conv = ctypes.CDLL(CONV_SO)
+        _conv.ref_op_tinv.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_int, ctypes.c_int]
+        _conv.ref_set_threads.argtypes = [ctypes.c_int]
+    return _conv
+
+
+def ref_threed_two(n, kernel):
+    """threed_two(-n:n-1)^3 (SD/sincproject.f90:110,182-187) from the kernel
+    on offsets -(n-1)..n-1: offset o at index o + n, zero at offset -n."""
+    t = np.zeros((2 * n, 2 * n, 2 * n), dtype=np.complex128, order="F")
+    t[1:, 1:, 1:] = kernel
+    return t
+
+
+def ref_op_tinv(n, kernel, den, circsize=None, threads=None):
+    """op_tinv through the reference's compiled circ3d_sub: den (n^3, allsize)
+    -> twoereduced (n^3, allsize), M = 2n.  None if oracle/_ref is not built."""
+    cl = conv_lib()
+    if cl is None:
+        return None
+    if threads is not None:
+        cl.ref_set_threads(int(threads))
+    den = np.asfortranarray(den, dtype=np.complex128)
+    if den.ndim == 1:
+        den = den.reshape(-1, 1, order="F")
+    allsize = den.shape[1]
+    circsize = allsize if circsize is None else circsize
+    if circsize <= 0 or allsize % circsize:
+        raise ValueError("SIZE ERROR OP_TINV")
+    t = ref_threed_two(n, kernel)
+    red = np.zeros_like(den, order="F")
+    _run_big_stack(lambda: cl.ref_op_tinv(n, t.ctypes.data, den.ctypes.data, red.ctypes.data, allsize, circsize),
+                   3 * (2 * n) ** 3 * circsize * 16)
+    return red
+
+
+def _run_big_stack(fn, automatic_bytes):
+    """circ3d_sub's AUTOMATIC arrays ffmat, ffvec, ffprod (SD/toeplitz_subs.f90:567-568)
+    live on the calling thread's stack in the amdflang build (the reference's
+    notes ask for `ulimit -s unlimited`): run the call on a thread whose stack
+    holds them."""
+    import threading
+    err = []
+
+    def body():
+        try:
+            fn()
+        except BaseException as e:  # pragma: no cover - re-raised below
+            err.append(e)
+
+    old = threading.stack_size()
+    threading.stack_size(max(1 << 24, int(automatic_bytes * 1.25) + (64 << 20)))
+    try:
+        th = threading.Thread(target=body)
+        th.start()
+    finally:
+        threading.stack_size(old)
+    th.join()
+    if err:
+        raise err[0]
 
 
 def _p(a):

@@ -184,6 +184,34 @@ def test_op_tinv_delta_linearity_conjugation(orc):
     assert relerr(orc.op_tinv(n, kern, np.conj(x)), np.conj(vx)) < 1e-14
 
 
+@pytest.mark.parametrize("n", [5, 7, 10])
+def test_op_tinv_reference_golden(orc, n):
+    """The oracle's op_tinv against what the reference's own circ3d_sub /
+    myzfft3d / DFFTPACK (compiled into oracle/_ref) returned for the same
+    seeded kernel and densities (tests/golden/ref_op_tinv.npz), at the
+    reference's M = 2n and at other circulant edges."""
+    g = np.load(os.path.join(GOLD, "ref_op_tinv.npz"))
+    kern, den, out = g["kern_%d" % n], g["den_%d" % n], g["out_%d" % n]
+    assert relerr(orc.op_tinv(n, kern, den), out) < 1e-14
+    assert relerr(orc.op_tinv(n, kern, den, M=2 * n + 3), out) < 1e-14
+    assert relerr(orc.op_tinv(n, kern, den, circsize=1), out) < 1e-14
+
+
+def test_op_tinv_vs_reference_build(orc):
+    """The oracle against the reference's convolution built from its sources
+    (oracle/_ref/libref_conv.so) on pair densities of seeded orbitals, with
+    circsize chunks as op_tinv_notscaled makes them."""
+    if orc.conv_lib() is None:
+        pytest.skip("oracle/_ref not built (reference sources not mounted)")
+    from sincdvr import inputs
+    for n, dx, N in [(12, 0.4, 3), (21, 0.4, 4)]:
+        phi = inputs.gaussian_orbitals(n, dx, N, 3, seed=n)
+        den = np.asfortranarray(phi * np.conj(phi[:, [1]]))
+        kern = orc.coulomb_kernel(n, dx)
+        ref = orc.ref_op_tinv(n, kern, den, circsize=1 if n == 12 else N)
+        assert relerr(orc.op_tinv(n, kern, den), ref) < 1e-14
+
+
 def test_op_tinv_size_error(orc):
     with pytest.raises(ValueError):
         orc.op_tinv(3, np.zeros((5, 5, 5)), np.zeros((27, 3), dtype=complex), circsize=2)
