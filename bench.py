@@ -93,7 +93,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(n, dx, phi, cores, nbatch_ref=1, nbatch_port=2):
+def cpu_baseline(n, dx, phi, cores, nbatch_ref=4, nbatch_port=4):
     """The reference's own CPU convolution -- circ3d_sub, myzfft3d and the
     DFFTPACK backend of myzfft1d compiled from the reference sources into
     oracle/_ref/libref_conv.so, framed as op_tinv_notscaled -- on `nbatch_ref`

@@ -165,6 +165,7 @@ struct XCfg {
 #ifndef SDV_YZ_TW_REGS
 #define SDV_YZ_TW_REGS 0
 #endif
+
 constexpr size_t yz_base_bytes(int R, int es) {
   return (size_t)(3 * R + (R & 1) + R * (R | 1)) * es;  // tables (2R + R + pad) and plane, bytes
 }
@@ -379,7 +380,7 @@ template <typename T, int P, int Q, bool GUARD, typename V, bool FULL = false>
 __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, unsigned char* smem_raw) {
   constexpr int R = P * Q, M = 2 * R, pitch = R | 1;
   typedef YZCfg<T, P, Q> C;
-  constexpr int G = C::G, GPW = C::GPW, NT = C::NT, LPG = C::LPG, NGB = C::NGB;
+  constexpr int G = C::G, GPW = C::GPW, NT = C::NT;
   constexpr bool KEEP = sizeof(T) == 4;
   // element (ky position p, row z) at Y[p * SP + z * SR]: FP64 keeps a
   // column contiguous (its column FFTs are the register-bound phase), FP32
@@ -390,21 +391,18 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   cplx<T>* Y = tw + a.lds_y;
   const int t = (!GUARD || threadIdx.x < NT) ? (int)threadIdx.x : (1 << 29);
   const int lane = t & 63, gi = lane / G, li = lane - gi * G;
-  const int grp = gi < GPW ? (t >> 6) * GPW + gi : (1 << 28);  // lines grp, grp + NGB, ...
+  const int grp = gi < GPW ? (t >> 6) * GPW + gi : (1 << 28);  // the one line (row, then column) of this group
   const int n = FULL ? R : a.n;
+  constexpr bool KH_LDS = yz_khat_in_lds(R, (int)sizeof(cplx<T>));
+  T* khs = reinterpret_cast<T*>(Y + R * pitch);
+  constexpr int PR = yz_park_rows(R, (int)sizeof(cplx<T>));
+  cplx<T>* park = Y + R * pitch;
+
+  // this plane's x frequency and its spectrum slice
   const int cx = plane / R, posx = plane - cx * R;
   const int kx = 2 * (posx / Q + P * (posx % Q)) + cx;
-  const int kkx = min(kx, M - kx);
-  constexpr bool KH_LDS = yz_khat_in_lds(R, (int)sizeof(cplx<T>));
-  const T* khg = a.khat + (size_t)kkx * (R + 1) * (R + 1);
-  T* khs = reinterpret_cast<T*>(Y + R * pitch);
-  const T* kh = KH_LDS ? khs : khg;
-  for (int i = t; i < M; i += NT) tw[i] = a.tw[i];
-  for (int i = t; i < R; i += NT) {
-    const int aa = i / P, bb = i - aa * P;
-    T2[i] = a.tw[(2 * aa * bb) % M];
-  }
-  if (KH_LDS) {  // every load in flight before the first store
+  const T* khg = a.khat + (size_t)min(kx, M - kx) * (R + 1) * (R + 1);
+  auto stage_kh = [&]() {  // FP32: every load in flight before the first store
     constexpr int KI = ((R + 1) * (R + 1) + NT - 1) / NT;
     T kv[KI];
 #pragma unroll
@@ -412,8 +410,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 #pragma unroll
     for (int i = 0; i < KI; ++i)
       if (t + i * NT < (R + 1) * (R + 1)) khs[t + i * NT] = kv[i];
-  }
-  cplx<T> o0[KEEP ? LPG : 1][P];  // cy = 0 output part (FP32)
+  };
 
   // Odd-channel twiddle w_M^(j1 Q + li) = w_M^li * w_2P^j1: the correctly
   // rounded table entry from LDS.  SDV_YZ_TW_REGS=1 forms it in FP64 as the
@@ -427,11 +424,22 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     if (TW_REG) return j1 == 0 ? wb : wb * mk((T)kTw<2 * P>.c[j1], (T)-kTw<2 * P>.s[j1]);
     return tw[j1 * Q + li];
   };
+  // FP32: the lane's step-1 twiddles read into registers before the DFT they
+  // follow (the compiler cannot tell the table from the plane, so a table
+  // read left between the plane stores waits for every store before it;
+  // 1.54 -> 1.51 ms at 81^3).  FP64 reads them in the store loop: its
+  // registers hold the column input and accumulator, and hoisting spills.
+  constexpr bool HOIST = sizeof(T) == 4;
+  auto t2_load = [&](cplx<T>* t2v, bool inv) {
+#pragma unroll
+    for (int k = 0; k < P; ++k) t2v[k] = inv ? T2[k * P + li] : T2[li * P + k];
+  };
 
-  // row input x[j1] = plane(z = g, y = j1 Q + li)
-  auto load_row = [&](int g, cplx<T>* x) {
+
+  // row input x[j1] = plane(z = g, y = j1 Q + li) of the plane at b
+  auto load_row = [&](const cplx<T>* b, int g, cplx<T>* x) {
     if (g < n && li < Q) {
-      const cplx<T>* grow = base + (size_t)g * n;
+      const cplx<T>* grow = b + (size_t)g * n;
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
         const int y = j1 * Q + li;
@@ -451,9 +459,12 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         const int y = j1 * Q + li;
         v[j1] = (cy && y < n) ? x[j1] * odd_tw(wb, j1) : x[j1];
       }
+      cplx<T> t2v[HOIST ? P : 1];
+      if constexpr (HOIST) t2_load(t2v, false);
       DFT<T, P, false>::run(v);
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1) yrow[(k1 * Q + li) * SP] = v[k1] * T2[li * P + k1];
+      for (int k1 = 0; k1 < P; ++k1)
+        yrow[(k1 * Q + li) * SP] = v[k1] * (HOIST ? t2v[HOIST ? k1 : 0] : T2[li * P + k1]);
     }
     wave_sync();
     if (rowok && li < P) {
@@ -468,6 +479,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   // both z channels of column c (ky position), spectrum of y channel cy
   auto zpass = [&](int cy, int c) {
     if (c >= R) return;
+    const T* kh = KH_LDS ? khs : khg;
     cplx<T>* ycol = Y + c * SP;  // column c: row z at ycol[z * SR]
     const int k1y = c / Q, k2y = c - (c / Q) * Q;
     const int ky = 2 * (k1y + P * k2y) + cy;
@@ -485,6 +497,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     // the inverse of both channels (T2[a P + b] is symmetric in a, b), so they
     // are read from LDS once per column
     constexpr bool T2C = sizeof(T) == 4 && P == Q;
+    constexpr bool H2 = HOIST && !T2C;
     cplx<T> t2c[T2C ? P : 1];
     if (T2C && li < Q) {
 #pragma unroll
@@ -500,15 +513,19 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         const cplx<T> wb = odd_base();
 #pragma unroll
         for (int j1 = 0; j1 < P; ++j1) v[j1] = cz ? u[j1] * odd_tw(wb, j1) : u[j1];
+        cplx<T> t2v[H2 ? P : 1];
+        if constexpr (H2) t2_load(t2v, false);
         DFT<T, P, false>::run(v);
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * t2f(k1);
+        for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * (H2 ? t2v[H2 ? k1 : 0] : t2f(k1));
       }
       wave_sync();
       if (li < P) {  // step 2, spectral multiply, inverse step 1
         cplx<T> w[Q];
 #pragma unroll
         for (int j2 = 0; j2 < Q; ++j2) w[j2] = ycol[(li * Q + j2) * SR];
+        cplx<T> t2v[H2 ? Q : 1];
+        if constexpr (H2) t2_load(t2v, true);
         DFT<T, Q, false>::run(w);
 #pragma unroll
         for (int k2 = 0; k2 < Q; ++k2) {
@@ -517,7 +534,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         }
         DFT<T, Q, true>::run(w);
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) ycol[(li * Q + j2) * SR] = w[j2] * conj(t2i(j2));
+        for (int j2 = 0; j2 < Q; ++j2) ycol[(li * Q + j2) * SR] = w[j2] * conj(H2 ? t2v[H2 ? j2 : 0] : t2i(j2));
       }
       wave_sync();
       if (li < Q) {  // inverse step 2; the odd channel carries w_M^-z
@@ -550,12 +567,14 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     const bool rowok = g < n;
     cplx<T>* yrow = Y + g * SR;
     if (rowok && li < P) {
-      cplx<T> u[Q];
+      cplx<T> u[Q], t2v[HOIST ? Q : 1];
 #pragma unroll
       for (int k2 = 0; k2 < Q; ++k2) u[k2] = yrow[(li * Q + k2) * SP];
+      if constexpr (HOIST) t2_load(t2v, true);
       DFT<T, Q, true>::run(u);
 #pragma unroll
-      for (int j2 = 0; j2 < Q; ++j2) yrow[(li * Q + j2) * SP] = u[j2] * conj(T2[j2 * P + li]);
+      for (int j2 = 0; j2 < Q; ++j2)
+        yrow[(li * Q + j2) * SP] = u[j2] * conj(HOIST ? t2v[HOIST ? j2 : 0] : T2[j2 * P + li]);
     }
     wave_sync();
     if (rowok && li < Q) {
@@ -569,86 +588,82 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     }
   };
 
-  __syncthreads();  // tables
-#pragma unroll
-  for (int r = 0; r < LPG; ++r) {
-    cplx<T> x[P];
-    load_row(grp + r * NGB, x);
-    yfwd(0, grp + r * NGB, x);
+  for (int i = t; i < M; i += NT) tw[i] = a.tw[i];
+  for (int i = t; i < R; i += NT) {
+    const int aa = i / P, bb = i - aa * P;
+    T2[i] = a.tw[(2 * aa * bb) % M];
   }
+  if (KH_LDS) stage_kh();
+  cplx<T> x[P];
+  load_row(base, grp, x);  // in flight during the table stores and the barrier
+  __syncthreads();         // tables (and the spectrum slice)
+  yfwd(0, grp, x);
   __syncthreads();
-#pragma unroll
-  for (int r = 0; r < LPG; ++r) zpass(0, grp + r * NGB);
+  zpass(0, grp);
   __syncthreads();
+  const int g = grp;
+  cplx<T> o[P], keep[P];
+  yinv(0, g, o);
+  load_row(base, g, x);  // the cy = 1 input: the same rows again, mostly from L2
+  if (KEEP) {
 #pragma unroll
-  for (int r = 0; r < LPG; ++r) {
-    const int g = grp + r * NGB;
-    cplx<T> o[P], x[P];
-    yinv(0, g, o);
-    load_row(g, x);
-    if (KEEP) {
-#pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) o0[KEEP ? r : 0][j1] = o[j1];
-      yfwd(1, g, x);
-    } else {
-      yfwd(1, g, x);  // consumes x before its HBM slots take the parked cy = 0 output
-      constexpr int PR = yz_park_rows(R, (int)sizeof(cplx<T>));
-      cplx<T>* park = Y + R * pitch;
-      if (g < PR) {
-        if (li < Q) {
-#pragma unroll
-          for (int j1 = 0; j1 < P; ++j1) park[(g * Q + li) * P + j1] = o[j1];
-        }
-      } else if (g < n && li < Q) {
-#pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) {
-          const int y = j1 * Q + li;
-          if (y < n) base[(size_t)g * n + y] = o[j1];
-        }
-      }
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < LPG; ++r) zpass(1, grp + r * NGB);
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < LPG; ++r) {
-    const int g = grp + r * NGB;
-    cplx<T> o[P], pk[P];
-    constexpr int PR = yz_park_rows(R, (int)sizeof(cplx<T>));
-    if (!KEEP && g < PR) {
+    for (int j1 = 0; j1 < P; ++j1) keep[j1] = o[j1];  // cy = 0 output part, in registers (FP32)
+    yfwd(1, g, x);
+  } else {
+    yfwd(1, g, x);  // consumes x before its HBM slots take the parked cy = 0 output
+    if (g < PR) {
       if (li < Q) {
 #pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) pk[j1] = (Y + R * pitch)[(g * Q + li) * P + j1];
+        for (int j1 = 0; j1 < P; ++j1) park[(g * Q + li) * P + j1] = o[j1];
       }
-    } else if (!KEEP && g < n && li < Q) {  // the parked cy = 0 part, in flight during the y-inverse
+    } else if (g < n && li < Q) {
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
         const int y = j1 * Q + li;
-        pk[j1] = y < n ? base[(size_t)g * n + y] : mk((T)0, (T)0);
+        if (y < n) base[(size_t)g * n + y] = o[j1];
       }
     }
-    yinv(1, g, o);
-    if (g < n && li < Q) {
+  }
+  __syncthreads();
+  zpass(1, grp);
+  __syncthreads();
+  if (!KEEP) {
+    if (g < PR) {
+      if (li < Q) {
+#pragma unroll
+        for (int j1 = 0; j1 < P; ++j1) keep[j1] = park[(g * Q + li) * P + j1];
+      }
+    } else if (g < n && li < Q) {  // the parked cy = 0 part, in flight during the y-inverse
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
         const int y = j1 * Q + li;
-        if (y < n) base[(size_t)g * n + y] = (KEEP ? o0[KEEP ? r : 0][j1] : pk[j1]) + o[j1];
+        keep[j1] = y < n ? base[(size_t)g * n + y] : mk((T)0, (T)0);
       }
+    }
+  }
+  yinv(1, g, o);
+  if (g < n && li < Q) {
+#pragma unroll
+    for (int j1 = 0; j1 < P; ++j1) {
+      const int y = j1 * Q + li;
+      if (y < n) base[(size_t)g * n + y] = keep[j1] + o[j1];
     }
   }
 }
 
+// Logical order plane-major, pairs fastest, one range per XCD: an XCD works
+// on a few x-frequency planes and keeps their spectrum slices in its L2 (FP32
+// 81^3: 1.60 -> 1.57 ms; FP64 unchanged).  One plane per workgroup: a
+// persistent form (the CU's workgroup walking a strided share of its XCD's
+// range, the plane body in a function of its own) measured 2.83 -> 3.20 ms in
+// FP64 at 81^3, the hardware's dynamic workgroup dispatch balancing better.
 template <typename T, int P, int Q, bool FULL>
 __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  // logical order plane-major, pairs fastest, one range per XCD: an XCD
-  // works on a few x-frequency planes and keeps their spectrum slices in its
-  // L2 (FP32 81^3: 1.60 -> 1.57 ms; FP64 unchanged)
   const unsigned L = xcd_span(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
   const unsigned plane = L / gridDim.y, pair = L % gridDim.y;
-  body_yz<T, P, Q, false, KArgs<T>, FULL>(a, a.X + ((size_t)pair * 2 * (P * Q) + plane) * a.ld, plane, smem_raw);
+  body_yz<T, P, Q, false, KArgs<T>, FULL>(a, a.X + ((size_t)pair * 2 * (P * Q) + plane) * a.ld, (int)plane,
+                                         smem_raw);
 }
 
 // ---------------------------------------------------------------- x inverse
