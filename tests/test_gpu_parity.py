@@ -166,6 +166,36 @@ def test_mean_field(orc, torch_dev, w21):
     assert relerr(out, orc.mult_reducedpot(2, 4, w21["phi_h"], rp[:, :, 1:4])) < 1e-13
 
 
+@pytest.mark.parametrize("n,N", [(41, 10), (9, 17), (81, 16)])
+def test_mean_field_matrix_cores(orc, torch_dev, n, N):
+    """get_reducedpot0's GEMM on the FP64 matrix cores: against the oracle
+    (41^3 with 10 orbitals; 9^3 with 17 orbitals, N^2 = 289 columns, two
+    column groups with a ragged one) or, at 81^3 with 16 orbitals, on 4096
+    sampled grid rows (including the ragged last row tile) against numpy."""
+    torch = torch_dev
+    from sincdvr import SincDVR
+    say("get_reducedpot n=%d N=%d" % (n, N))
+    rng = np.random.default_rng(1000 * n + N)
+    g = np.asfortranarray(rng.standard_normal((N, N, N, N)) + 1j * rng.standard_normal((N, N, N, N)))
+    s = SincDVR(n, 0.3, N)
+    if n < 81:
+        tr = np.asfortranarray(rng.standard_normal((n ** 3, N, N)) + 1j * rng.standard_normal((n ** 3, N, N)))
+        assert relerr(s.get_reducedpot(tr, g), orc.reducedpot(tr, g)) < 1e-13
+        return
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(n)
+    trd = torch.complex(torch.randn((N, N, n ** 3), dtype=torch.float64, device="cuda", generator=gen),
+                        torch.randn((N, N, n ** 3), dtype=torch.float64, device="cuda", generator=gen))
+    gd = torch.from_numpy(np.ascontiguousarray(g.reshape(N * N, N * N, order="F").T)).cuda()
+    rp = s.get_reducedpot(trd, gd)          # (N, N, n^3): reducedpot(n^3, N^2) in Fortran order
+    torch.cuda.synchronize()
+    rows = np.unique(np.concatenate([rng.integers(0, n ** 3, 4000), np.arange(n ** 3 - 96, n ** 3)]))
+    A = trd.reshape(N * N, n ** 3)[:, rows].cpu().numpy().T           # (rows, N^2)
+    ref = A @ g.reshape(N * N, N * N, order="F")
+    got = rp.reshape(N * N, n ** 3)[:, rows].cpu().numpy().T
+    assert relerr(got, ref) < 1e-13
+
+
 def test_twoe_matel_grid41(orc, torch_dev):
     """41^3 grid, 10 orbitals (55 unique / 100 ordered pairs), hermnorm."""
     from sincdvr import SincDVR, inputs

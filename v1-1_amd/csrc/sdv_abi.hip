@@ -330,7 +330,7 @@ int sdv_free(sdv_ctx* c) {
   hipFree(c->dip);
   for (sdv_ctx::Buf* b : {&c->work, &c->scratch, &c->gout, &c->h_in1, &c->h_in2, &c->h_out1, &c->h_out2, &c->ob_ke,
                           &c->ob_scr, &c->h_pot, &c->ar_w, &c->ar_rp, &c->ar_s, &c->ar_s2, &c->ar_dots,
-                          &c->h_small, &c->h_out3})
+                          &c->h_small, &c->h_out3, &c->rp_bt})
     hipFree(b->p);
   for (auto& kv : c->plans) hipFree(kv.second.dev);
   for (auto& r : c->prof.recs) {
@@ -1138,7 +1138,8 @@ int sdv_mult_reducedpot_dev(sdv_ctx* c, int first, int last, const void* inspfs,
 
 int sdv_get_reducedpot_dev(sdv_ctx* c, const void* tr, const void* twoden, void* rp, void* stream) {
   SDV_REQUIRE(c && tr && twoden && rp, SDV_ERR_ARG, "null argument");
-  return reducedpot_launch(c->n3, c->N, tr, twoden, rp, (hipStream_t)stream);
+  SDV_TRY(ensure(c->rp_bt, reducedpot_scratch_bytes(c->N)));
+  return reducedpot_launch(c->n3, c->N, tr, twoden, rp, c->rp_bt.p, (hipStream_t)stream);
 }
 
 // ---- host-pointer (Fortran drop-in) variants: stage, run, copy back, sync.

@@ -48,6 +48,7 @@ struct sdv_ctx {
   };
   Buf work, scratch, gout, h_in1, h_in2, h_out1, h_out2, ob_ke, ob_scr, h_pot;
   Buf ar_w, ar_rp, ar_s, ar_s2, ar_dots, h_small, h_out3;  // orbital operator (actreduced00)
+  Buf rp_bt;  // get_reducedpot: twoden transposed
 };
 
 namespace sdv {

@@ -204,7 +204,8 @@ int orb_sum_launch(int n, int ncols, const void* ke, const double* pot, const vo
                    const void* dip, const double* tdpot, void* s, hipStream_t st);
 
 // Mean field
-int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, hipStream_t st);
+size_t reducedpot_scratch_bytes(int N);  // twoden transposed, N^4 complex
+int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, void* scratch, hipStream_t st);
 int mult_reducedpot_launch(int n3, int N, int nout, const void* inspfs, const void* rp, void* out,
                            hipStream_t st);
 

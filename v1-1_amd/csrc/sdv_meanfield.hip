@@ -2,52 +2,12 @@
 //
 //   get_reducedpot0 (S/mean.f90:556-558):
 //     reducedpot(x, c) = sum_k twoereduced(x, k) * twoden(k, c),  k, c in [0, N^2)
+//     (a complex GEMM, on the FP64 matrix cores)
 //   mult_reducedpot (SD/coreproject.f90:1200-1235):
 //     out(x, i) = sum_k reducedpot(x, k, i) * in(x, k)
 #include "sdv_internal.h"
 
 namespace sdv {
-
-constexpr int kRpCols = 16;   // output columns per workgroup
-constexpr int kRpKc = 64;     // twoden rows staged per step
-
-// Each thread owns one grid point x and kRpCols output columns; the
-// twoden tile is staged in LDS and read as broadcasts.
-__global__ void __launch_bounds__(256) k_reducedpot(int n3, int NN, const z64* __restrict__ tr,
-                                                     const z64* __restrict__ g, z64* __restrict__ out) {
-  __shared__ z64 gs[kRpKc][kRpCols];
-  const int x = blockIdx.x * 256 + threadIdx.x;
-  const int c0 = blockIdx.y * kRpCols;
-  z64 acc[kRpCols];
-#pragma unroll
-  for (int j = 0; j < kRpCols; ++j) acc[j] = mk(0.0, 0.0);
-  for (int k0 = 0; k0 < NN; k0 += kRpKc) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kRpKc * kRpCols; e += 256) {
-      const int kk = e / kRpCols, j = e - kk * kRpCols;
-      const int k = k0 + kk, c = c0 + j;
-      gs[kk][j] = (k < NN && c < NN) ? g[(size_t)k + (size_t)NN * c] : mk(0.0, 0.0);
-    }
-    __syncthreads();
-    if (x < n3) {
-      const int kmax = min(kRpKc, NN - k0);
-      for (int kk = 0; kk < kmax; ++kk) {
-        const z64 t = tr[(size_t)x + (size_t)n3 * (k0 + kk)];
-#pragma unroll
-        for (int j = 0; j < kRpCols; ++j) {
-          const z64 gv = gs[kk][j];
-          acc[j].re = fma(t.re, gv.re, fma(-t.im, gv.im, acc[j].re));
-          acc[j].im = fma(t.re, gv.im, fma(t.im, gv.re, acc[j].im));
-        }
-      }
-    }
-  }
-  if (x < n3) {
-#pragma unroll
-    for (int j = 0; j < kRpCols; ++j)
-      if (c0 + j < NN) out[(size_t)x + (size_t)n3 * (c0 + j)] = acc[j];
-  }
-}
 
 __global__ void k_mult_reducedpot(int n3, int N, const z64* __restrict__ in, const z64* __restrict__ rp,
                                   z64* __restrict__ out) {
@@ -63,10 +23,89 @@ __global__ void k_mult_reducedpot(int n3, int N, const z64* __restrict__ in, con
   out[(size_t)x + (size_t)n3 * i] = acc;
 }
 
-int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, hipStream_t st) {
+// ---- the same product on the FP64 matrix cores
+// reducedpot (M x K) . twoden (K x K), M = n^3, K = N^2, as four real
+// products per complex one:  Re += Re A Re B - Im A Im B,  Im += Re A Im B +
+// Im A Re B, on v_mfma_f64_16x16x4f64 (A operand: lane l holds row l % 16,
+// k l / 16; B operand: k l / 16, column l % 16; accumulator element v of lane
+// l is row 4 v + l / 16, column l % 16, as checked on gfx950).  A workgroup
+// of eight waves computes 64 rows x 256 columns (wave: 32 rows x 64
+// columns, 2 x 4 tiles of 16 x 16, 16 accumulators); the operands come
+// straight from global memory: A's 16 rows of one k are 256 contiguous bytes,
+// and twoden is first transposed (k-major, 1 MB at 16 orbitals, L2 resident)
+// so that B's 16 columns of one k are too.  8 n^3 N^4 real flops
+// (0.28 TFLOP at 81^3 with 16 orbitals), compute-bound on the matrix cores.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kRmRows = 64, kRmCols = 256, kRmThreads = 512;
+
+__global__ void k_transpose_z64(int K, const z64* __restrict__ g, z64* __restrict__ gt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // gt index: k * K + c
+  if (i >= (long long)K * K) return;
+  const int k = (int)(i / K), c = (int)(i - (long long)k * K);
+  gt[i] = g[(size_t)k + (size_t)K * c];
+}
+
+__global__ void __launch_bounds__(kRmThreads) k_reducedpot_mfma(int M, int K, const z64* __restrict__ A,
+                                                               const z64* __restrict__ Bt, z64* __restrict__ C) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int lr = l & 15, lk = l >> 4;
+  const long long x0 = (long long)blockIdx.x * kRmRows + 32 * (w & 1);
+  const int c0 = blockIdx.y * kRmCols + 64 * (w >> 1);
+  f64x4 are[2][4], aim[2][4];
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      are[rt][ct] = f64x4{0, 0, 0, 0};
+      aim[rt][ct] = f64x4{0, 0, 0, 0};
+    }
+#pragma unroll 2
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int k = k0 + lk;
+    z64 a[2], b[4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+      const long long x = x0 + 16 * rt + lr;
+      a[rt] = (x < M && k < K) ? A[x + (long long)M * k] : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      const int c = c0 + 16 * ct + lr;
+      b[ct] = (c < K && k < K) ? Bt[(long long)k * K + c] : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].re, b[ct].re, are[rt][ct], 0, 0, 0);
+        are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[rt].im, b[ct].im, are[rt][ct], 0, 0, 0);
+        aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].re, b[ct].im, aim[rt][ct], 0, 0, 0);
+        aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].im, b[ct].re, aim[rt][ct], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+      const int c = c0 + 16 * ct + lr;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const long long x = x0 + 16 * rt + 4 * v + lk;
+        if (x < M && c < K) C[x + (long long)M * c] = mk(are[rt][ct][v], aim[rt][ct][v]);
+      }
+    }
+}
+
+size_t reducedpot_scratch_bytes(int N) { return (size_t)N * N * N * N * sizeof(z64); }
+
+int reducedpot_launch(int n3, int N, const void* tr, const void* twoden, void* out, void* scratch, hipStream_t st) {
   const int NN = N * N;
-  hipLaunchKernelGGL(k_reducedpot, dim3((n3 + 255) / 256, (NN + kRpCols - 1) / kRpCols), dim3(256), 0, st, n3, NN,
-                     (const z64*)tr, (const z64*)twoden, (z64*)out);
+  const long long nt = (long long)NN * NN;
+  hipLaunchKernelGGL(k_transpose_z64, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st, NN,
+                     (const z64*)twoden, (z64*)scratch);
+  SDV_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_reducedpot_mfma, dim3((unsigned)((n3 + kRmRows - 1) / kRmRows), (NN + kRmCols - 1) / kRmCols),
+                     dim3(kRmThreads), 0, st, n3, NN, (const z64*)tr, (const z64*)scratch, (z64*)out);
   SDV_HIP_CHECK(hipGetLastError());
   return SDV_OK;
 }

@@ -14,6 +14,19 @@
 ! sincdvr_gpu_init, which the project's init_project would call once.
 ! A nonzero library status prints the library message and stops, as the
 ! reference's CFLST does (S/Definitions.ALL:45).
+!
+! Timing side effects, as the reference's routines have them: op_tinv adds
+! its elapsed milliseconds to the caller's times4 counter (the F.T. column;
+! the padded-array fill and the MPI exchanges that times1/3/5 count have no
+! counterpart here, and the non-MPI reference leaves fttimes alone);
+! call_twoe_matel00 keeps the reference's per-call times(1:7) and, with
+! notiming = 0, writes timingdir/twoematel.time.dat: the header line on the
+! first call, then one line per call (SD/coreproject.f90:1103-1142) with the
+! columns etc, 1den, 1mpi, F.T., 2mpi, dot, ALL and the seven fttimes (zero
+! without MPI), in milliseconds.  The F.T. and dot columns come from the
+! library's per-kernel event timing (sdv_profile, on only while notiming = 0);
+! the density formation runs inside the x-forward kernel, so 1den is 0 and
+! the densities count under F.T.; etc is the rest of the call (transfers).
 
 module sincdvr_capi
   use, intrinsic :: iso_c_binding
@@ -69,6 +82,17 @@ module sincdvr_capi
        complex(c_double_complex), intent(out) :: out(*)
        integer(c_int), value :: howmany
      end function sdv_mult_ke
+     integer(c_int) function sdv_profile(ctx, enable) bind(c, name="sdv_profile")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: enable
+     end function sdv_profile
+     integer(c_int) function sdv_profile_read(ctx, ms, cnt) bind(c, name="sdv_profile_read")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ms(*)
+       integer(c_int), intent(out) :: cnt(*)
+     end function sdv_profile_read
      integer(c_int) function sdv_mult_reducedpot(ctx, first, last, in, out, rp) bind(c, name="sdv_mult_reducedpot")
        import :: c_int, c_ptr, c_double_complex
        type(c_ptr), value :: ctx
@@ -81,7 +105,24 @@ module sincdvr_capi
   type(c_ptr), save :: sdv_ctx_ptr = c_null_ptr
   integer, save :: sdv_totpoints = -1, sdv_numspf = -1
 
+  integer, parameter :: sdv_nstages = 7     ! SDV_NSTAGES: xfwd, yz, xinv, matel assemble/gemm/reduce, ke
+
 contains
+
+  ! Wall clock in milliseconds of the day, the unit of the reference's timing
+  ! counters (clockmod's myclock, S/parameters.f90:30-35).
+  integer function sdv_clock_ms()
+    integer :: v(8)
+    call date_and_time(values=v)
+    sdv_clock_ms = v(8) + 1000 * (v(7) + 60 * (v(6) + 60 * v(5)))
+  end function sdv_clock_ms
+
+  ! timingdir(1:getlen(timingdir)): up to the first blank (S/utils.f90:38-53)
+  integer function sdv_dirlen(dir)
+    character(*), intent(in) :: dir
+    sdv_dirlen = index(dir, " ") - 1
+    if (sdv_dirlen < 0) sdv_dirlen = len(dir)
+  end function sdv_dirlen
 
   subroutine sdv_check(rc, where)
     integer(c_int), intent(in) :: rc
@@ -146,7 +187,10 @@ contains
     integer, intent(inout) :: times1, times3, times4, times5, fttimes(10)
     complex*16, intent(in) :: twoeden03(sdv_totpoints, allsize)
     complex*16, intent(out) :: twoereduced(sdv_totpoints, allsize)
+    integer :: itime
+    itime = sdv_clock_ms()
     call sdv_check(sdv_op_tinv(sdv_ctx_ptr, twoeden03, twoereduced, allsize, circsize), "op_tinv")
+    times4 = times4 + sdv_clock_ms() - itime
   end subroutine op_tinv
 end module tinvsubmod
 
@@ -171,11 +215,45 @@ subroutine call_twoe_matel00(lowspf, highspf, inspfs10, inspfs20, twoematel, two
        twoereduced(sdv_totpoints, sdv_numspf, lowspf:highspf)
   character, intent(in) :: timingdir*(*)
   type(c_ptr) :: p10, p20
-  if (highspf < lowspf) return
+  integer, save :: xcount = 0
+  integer :: times(10), fttimes(10), firsttime, myiostat
+  real(c_double) :: ms(sdv_nstages)
+  integer(c_int) :: cnt(sdv_nstages)
+  if (highspf < lowspf) return      ! SD/coreproject.f90:969-972
+  times(:) = 0; fttimes(:) = 0      ! per call, as the reference zeroes them
+  firsttime = sdv_clock_ms()
+  if (notiming == 0) call sdv_check(sdv_profile(sdv_ctx_ptr, 1), "sdv_profile")
   p10 = c_loc(inspfs10)
   p20 = c_loc(inspfs20)     ! the same address when the caller passes one array twice
   call sdv_check(sdv_call_twoe_matel00(sdv_ctx_ptr, lowspf, highspf, p10, p20, c_loc(twoematel), &
        c_loc(twoereduced)), "call_twoe_matel00")
+  times(7) = sdv_clock_ms() - firsttime
+  if (notiming == 0) then
+     call sdv_check(sdv_profile_read(sdv_ctx_ptr, ms, cnt), "sdv_profile_read")
+     call sdv_check(sdv_profile(sdv_ctx_ptr, 0), "sdv_profile")
+     times(4) = nint(ms(1) + ms(2) + ms(3))            ! F.T.: the three convolution kernels
+     times(6) = nint(ms(4) + ms(5) + ms(6))            ! dot: the matrix-element reduction
+     times(1) = max(0, times(7) - times(4) - times(6))  ! etc
+  end if
+  xcount = xcount + 1
+  if (notiming == 0) then
+     if (xcount == 1) then
+        open(853, file=timingdir(1:sdv_dirlen(timingdir))//"/twoematel.time.dat", status="unknown", &
+             iostat=myiostat)
+        if (myiostat == 0) then
+           write(853, '(100A11)') "etc", "1den", "1mpi", "F.T.", "2mpi", "dot", "ALL"
+           close(853)
+        end if
+     end if
+     open(853, file=timingdir(1:sdv_dirlen(timingdir))//"/twoematel.time.dat", status="unknown", &
+          position="append", iostat=myiostat)
+     if (myiostat /= 0) then
+        write(*, *) "opening twoematel timing sincdvr failed: ", timingdir(1:sdv_dirlen(timingdir))
+        stop 1
+     end if
+     write(853, '(100I11)') times(1:7), fttimes(1:7)
+     close(853)
+  end if
 end subroutine call_twoe_matel00
 
 
