@@ -136,6 +136,26 @@ def cpu_baseline(n, dx, phi, cores, nbatch_ref=4, nbatch_port=4):
                  "port": port}, **host)
 
 
+def bench_small_configs(torch, steps):
+    """Pair potentials/s of the single-set path on the 21^3 / 4-orbital and
+    41^3 / 10-orbital workloads (FP64 and mixed), beside the 81^3 headline."""
+    from sincdvr import SincDVR, inputs
+    res = {}
+    for key in ("grid21_4orb", "grid41_10orb"):
+        cfg = inputs.CONFIGS[key]
+        n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
+        phi = inputs.gaussian_orbitals_device(n, dx, N, cfg["ngauss"], cfg["seed"])
+        s = SincDVR(n, dx, N)
+        tr = torch.zeros((N, N, n ** 3), dtype=torch.complex128, device="cuda")
+        ent = {"grid": n, "orbitals": N, "unique_pairs": N * (N + 1) // 2, "circulant_edge": s.circulant_edge}
+        for prec, name in ((0, "fp64"), (1, "mixed")):
+            s.set_precision(prec)
+            t = time_steps(lambda: s.twoereduced(1, N, phi, out=tr), max(steps, 20), 3, torch)
+            ent[name] = {"pairs_per_s": N * (N + 1) // 2 / t, "ms_per_step": t * 1e3}
+        res[key] = ent
+    return res
+
+
 def bench_batched_81(torch, s, phi, npairs, steps):
     """The 81^3 / 16-orbital pairs through the batch API, mixed precision,
     replayed as one hipGraph per step."""
@@ -405,6 +425,8 @@ def main():
         bm = bench_batched_81(torch, s, phi, npairs, args.steps)
         bm["speedup_vs_fp64"] = bm["value"] * t64 / npairs
         out["batched_mixed_81"] = bm
+        print("[bench] 21^3 and 41^3 workloads", file=sys.stderr, flush=True)
+        out["smaller_grids"] = bench_small_configs(torch, args.steps)
         print("[bench] 64-set batch", file=sys.stderr, flush=True)
         out["batch"] = bench_batch(torch, args.quick)
         print("[bench] serving", file=sys.stderr, flush=True)

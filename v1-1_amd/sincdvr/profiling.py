@@ -11,7 +11,7 @@ summarize: reads rocprofv3 output (kernel-trace stats CSV, counter CSVs of a
      FETCH_SIZE pass and a WRITE_SIZE pass) and writes a short markdown summary
      and the traffic figure the benchmark reports.
 
-    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py --no-cpu --no-extra
+    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py --no-cpu --no-extra --steps 5 --warmup 2
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- python3 v1-1_amd/sincdvr/profiling.py run
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- python3 v1-1_amd/sincdvr/profiling.py run
     python3 v1-1_amd/sincdvr/profiling.py summarize OUT profiles
@@ -140,7 +140,7 @@ def summarize(out_dir, prof_dir, tag="pairpot_81_n16"):
     with open(os.path.join(prof_dir, f"{tag}_profile_summary.md"), "w") as f:
         f.write(f"# rocprofv3 summary: {tag}\n\n")
         f.write("Commands (repository root, on the GPU):\n\n")
-        f.write("    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py --no-cpu --no-extra\n")
+        f.write("    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py --no-cpu --no-extra --steps 5 --warmup 2\n")
         f.write("    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- python3 v1-1_amd/sincdvr/profiling.py run\n")
         f.write("    rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- python3 v1-1_amd/sincdvr/profiling.py run\n")
         f.write("    python3 v1-1_amd/sincdvr/profiling.py summarize OUT profiles\n\n")
@@ -182,6 +182,7 @@ def summarize_sq(out_dir, prof_dir):
         "| kernel | VALU insts | LDS insts | wait any / wave cycles | wait issue / wave cycles | "
         "LDS bank conflict / LDS active |",
         "|---|---:|---:|---:|---:|---:|"]
+    yz64 = None
     for p, label in ((0, "FP64"), (1, "FP32")):
         rows = _rows(os.path.join(out_dir, f"sq{p}", "**", "*counter_collection.csv"))
         c = {name: counter_per_dispatch(rows, name) for name in SQ_COUNTERS}
@@ -199,11 +200,25 @@ def summarize_sq(out_dir, prof_dir):
                 c["SQ_LDS_BANK_CONFLICT"].get(k, 0.0) / lds_act if lds_act else 0.0))
             if k.startswith("sdv::k_yz<") and wc:
                 busy = (c["SQ_INSTS_VALU"][k] / SIMDS) / (wc / (SIMDS * YZ_WAVES_PER_SIMD))
+                if p == 0:
+                    lds_act = c["SQ_LDS_IDX_ACTIVE"].get(k, 0.0)
+                    yz64 = {"valu_busy": busy, "waves_per_cu": 4 * YZ_WAVES_PER_SIMD,
+                            "lds_bank_conflict_frac": (c["SQ_LDS_BANK_CONFLICT"].get(k, 0.0) / lds_act
+                                                       if lds_act else None),
+                            "wait_any_frac": c["SQ_WAIT_ANY"].get(k, 0.0) / wc,
+                            "source": "rocprofv3 SQ counters (profiles/k_yz_81_n16_sq_counters.md)"}
         if busy is not None:
             lines.append("|  (VALU busy %s) | %.2f | | | | |" % (label, busy))
     os.makedirs(prof_dir, exist_ok=True)
     with open(os.path.join(prof_dir, "k_yz_81_n16_sq_counters.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
+    tj = os.path.join(prof_dir, "traffic_k_yz_81.json")
+    if yz64 and os.path.exists(tj):  # the limiter figures the benchmark reports beside the traffic
+        with open(tj) as f:
+            t = json.load(f)
+        t.update(yz64)
+        with open(tj, "w") as f:
+            json.dump(t, f, indent=1)
     return lines
 
 
