@@ -74,8 +74,6 @@ def ref_op_tinv(n, kernel, den, circsize=None, threads=None):
     cl = conv_lib()
     if cl is None:
         return None
-    if threads is not None:
-        cl.ref_set_threads(int(threads))
     den = np.asfortranarray(den, dtype=np.complex128)
     if den.ndim == 1:
         den = den.reshape(-1, 1, order="F")
@@ -85,8 +83,14 @@ def ref_op_tinv(n, kernel, den, circsize=None, threads=None):
         raise ValueError("SIZE ERROR OP_TINV")
     t = ref_threed_two(n, kernel)
     red = np.zeros_like(den, order="F")
-    _run_big_stack(lambda: cl.ref_op_tinv(n, t.ctypes.data, den.ctypes.data, red.ctypes.data, allsize, circsize),
-                   3 * (2 * n) ** 3 * circsize * 16)
+    def call():
+        # the OpenMP thread count is a per-thread setting: set it on the
+        # thread that runs the reference's parallel regions
+        if threads is not None:
+            cl.ref_set_threads(int(threads))
+        cl.ref_op_tinv(n, t.ctypes.data, den.ctypes.data, red.ctypes.data, allsize, circsize)
+
+    _run_big_stack(call, 3 * (2 * n) ** 3 * circsize * 16)
     return red
 
 
