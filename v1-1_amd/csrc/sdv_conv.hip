@@ -165,6 +165,9 @@ struct XCfg {
 #ifndef SDV_YZ_TW_REGS
 #define SDV_YZ_TW_REGS 0
 #endif
+#ifndef SDV_YZ_TWB
+#define SDV_YZ_TWB 3
+#endif
 
 constexpr size_t yz_base_bytes(int R, int es) {
   return (size_t)(3 * R + (R & 1) + R * (R | 1)) * es;  // tables (2R + R + pad) and plane, bytes
@@ -350,8 +353,10 @@ __global__ void __launch_bounds__(XThreads<T>::fwd) k_xfwd(KArgs<T> a) {
 //             scratch of its own FFTs;
 //   rows    : y-inverse of row z, this channel's output part.
 // Five workgroup barriers per plane.  The cy = 0 output part stays in
-// registers (FP32) or is parked (FP64, whose registers are needed for the
-// column FFTs): rows z < yz_park_rows in LDS after the plane, each lane in
+// registers (FP32, and FP64 on full grids) or is parked (FP64 with n < m,
+// whose registers are needed for the column FFTs; the LDS after the plane is
+// reserved for the parked rows on full grids too, where it goes unused):
+// rows z < yz_park_rows in LDS after the plane, each lane in
 // slots of its own, the other rows in the plane's own HBM slots, which the
 // same lanes have just read for the cy = 1 input; the parked values are loaded
 // before the last y-inverse, so their latency overlaps it.  Row pitches other
@@ -381,7 +386,10 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   constexpr int R = P * Q, M = 2 * R, pitch = R | 1;
   typedef YZCfg<T, P, Q> C;
   constexpr int G = C::G, GPW = C::GPW, NT = C::NT;
-  constexpr bool KEEP = sizeof(T) == 4;
+  // the cy = 0 output part stays in registers in FP32 and in FP64 on full
+  // grids (150 VGPRs at m = 81, no spill; 2.58 -> 2.45 ms at 81^3 against
+  // parking); FP64 with n < m parks it (keeping it there spills)
+  constexpr bool KEEP = sizeof(T) == 4 || FULL;
   // element (ky position p, row z) at Y[p * SP + z * SR]: FP64 keeps a
   // column contiguous (its column FFTs are the register-bound phase), FP32
   // keeps a row contiguous
@@ -434,6 +442,27 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 #pragma unroll
     for (int k = 0; k < P; ++k) t2v[k] = inv ? T2[k * P + li] : T2[li * P + k];
   };
+  // FP64 stores a step's K outputs v[k] * tab(k) with the table reads issued
+  // TWB at a time ahead of the stores they scale.  A read placed between two
+  // plane stores can only be issued after them (the table and plane offsets
+  // are runtime values, so the compiler must assume they alias) and its
+  // result waits for every LDS operation before it: one read per store costs
+  // a full LDS round trip per element.  Batches keep TWB twiddles live
+  // (hoisting all nine spills at 166 VGPRs).
+  constexpr int TWB = SDV_YZ_TWB;
+  auto mul_store = [&](auto kk, const cplx<T>* v, auto tab, auto st) {
+    constexpr int K = decltype(kk)::value;
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += TWB) {
+      cplx<T> w[TWB];
+#pragma unroll
+      for (int b = 0; b < TWB; ++b)
+        if (k0 + b < K) w[b] = tab(k0 + b);
+#pragma unroll
+      for (int b = 0; b < TWB; ++b)
+        if (k0 + b < K) st(k0 + b, v[k0 + b] * w[b]);
+    }
+  };
 
 
   // row input x[j1] = plane(z = g, y = j1 Q + li) of the plane at b
@@ -462,9 +491,13 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       cplx<T> t2v[HOIST ? P : 1];
       if constexpr (HOIST) t2_load(t2v, false);
       DFT<T, P, false>::run(v);
+      if constexpr (HOIST) {
 #pragma unroll
-      for (int k1 = 0; k1 < P; ++k1)
-        yrow[(k1 * Q + li) * SP] = v[k1] * (HOIST ? t2v[HOIST ? k1 : 0] : T2[li * P + k1]);
+        for (int k1 = 0; k1 < P; ++k1) yrow[(k1 * Q + li) * SP] = v[k1] * t2v[HOIST ? k1 : 0];
+      } else {
+        mul_store(std::integral_constant<int, P>{}, v, [&](int k1) { return T2[li * P + k1]; },
+                  [&](int k1, const cplx<T>& x) { yrow[(k1 * Q + li) * SP] = x; });
+      }
     }
     wave_sync();
     if (rowok && li < P) {
@@ -516,8 +549,13 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         cplx<T> t2v[H2 ? P : 1];
         if constexpr (H2) t2_load(t2v, false);
         DFT<T, P, false>::run(v);
+        if constexpr (HOIST) {
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * (H2 ? t2v[H2 ? k1 : 0] : t2f(k1));
+          for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * (H2 ? t2v[H2 ? k1 : 0] : t2f(k1));
+        } else {
+          mul_store(std::integral_constant<int, P>{}, v, t2f,
+                    [&](int k1, const cplx<T>& x) { ycol[(k1 * Q + li) * SR] = x; });
+        }
       }
       wave_sync();
       if (li < P) {  // step 2, spectral multiply, inverse step 1
@@ -533,8 +571,13 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
           w[k2] = scale(w[k2], kh[kcol + min(kz, M - kz)]);
         }
         DFT<T, Q, true>::run(w);
+        if constexpr (HOIST) {
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) ycol[(li * Q + j2) * SR] = w[j2] * conj(H2 ? t2v[H2 ? j2 : 0] : t2i(j2));
+          for (int j2 = 0; j2 < Q; ++j2) ycol[(li * Q + j2) * SR] = w[j2] * conj(H2 ? t2v[H2 ? j2 : 0] : t2i(j2));
+        } else {
+          mul_store(std::integral_constant<int, Q>{}, w, [&](int j2) { return conj(t2i(j2)); },
+                    [&](int j2, const cplx<T>& x) { ycol[(li * Q + j2) * SR] = x; });
+        }
       }
       wave_sync();
       if (li < Q) {  // inverse step 2; the odd channel carries w_M^-z
@@ -547,13 +590,11 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         for (int j1 = 0; j1 < P; ++j1) acc[j1] = acc[j1] + (cz ? v[j1] * conj(odd_tw(wb, j1)) : v[j1]);
       }
     };
-    if constexpr (sizeof(T) == 4) {  // FP32 has the registers for both channels unrolled
-      zchan(std::integral_constant<int, 0>{});
-      zchan(std::integral_constant<int, 1>{});
-    } else {
-#pragma unroll 1
-      for (int cz = 0; cz < 2; ++cz) zchan(cz);
-    }
+    // both channels unrolled (cz known at compile time): with a runtime
+    // channel loop FP64 branched around every odd-channel twiddle and held
+    // 166 VGPRs; unrolled it holds 114-148 (2.68 -> 2.58 ms at 81^3)
+    zchan(std::integral_constant<int, 0>{});
+    zchan(std::integral_constant<int, 1>{});
     if (li < Q) {
 #pragma unroll
       for (int j1 = 0; j1 < P; ++j1) {
@@ -572,9 +613,13 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       for (int k2 = 0; k2 < Q; ++k2) u[k2] = yrow[(li * Q + k2) * SP];
       if constexpr (HOIST) t2_load(t2v, true);
       DFT<T, Q, true>::run(u);
+      if constexpr (HOIST) {
 #pragma unroll
-      for (int j2 = 0; j2 < Q; ++j2)
-        yrow[(li * Q + j2) * SP] = u[j2] * conj(HOIST ? t2v[HOIST ? j2 : 0] : T2[j2 * P + li]);
+        for (int j2 = 0; j2 < Q; ++j2) yrow[(li * Q + j2) * SP] = u[j2] * conj(t2v[HOIST ? j2 : 0]);
+      } else {
+        mul_store(std::integral_constant<int, Q>{}, u, [&](int j2) { return conj(T2[j2 * P + li]); },
+                  [&](int j2, const cplx<T>& x) { yrow[(li * Q + j2) * SP] = x; });
+      }
     }
     wave_sync();
     if (rowok && li < Q) {
