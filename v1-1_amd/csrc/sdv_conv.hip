@@ -449,6 +449,8 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   // result waits for every LDS operation before it: one read per store costs
   // a full LDS round trip per element.  Batches keep TWB twiddles live
   // (hoisting all nine spills at 166 VGPRs).
+  // Every step-1 twiddle table has w^0 = 1 at index 0 (T2[l P] and T2[l]),
+  // so output 0 is stored unscaled: no read, no multiply.
   constexpr int TWB = SDV_YZ_TWB;
   auto mul_store = [&](auto kk, const cplx<T>* v, auto tab, auto st) {
     constexpr int K = decltype(kk)::value;
@@ -457,11 +459,22 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       cplx<T> w[TWB];
 #pragma unroll
       for (int b = 0; b < TWB; ++b)
-        if (k0 + b < K) w[b] = tab(k0 + b);
+        if (k0 + b < K && k0 + b > 0) w[b] = tab(k0 + b);
 #pragma unroll
       for (int b = 0; b < TWB; ++b)
-        if (k0 + b < K) st(k0 + b, v[k0 + b] * w[b]);
+        if (k0 + b < K) st(k0 + b, k0 + b == 0 ? v[0] : v[k0 + b] * w[b]);
     }
+  };
+  // acc + v * conj(w): in FP64 each component one expression (two fused
+  // multiply-adds, 4 VALU operations instead of 6); FP32 rounds the product
+  // first, as the mixed form's error figures were measured with (fused, the
+  // RMS error of the 41^3 c-norm diagonal pair potentials is unchanged but
+  // their largest single-element error went from 8.1e-8 to 1.5e-7)
+  auto macc_conj = [](const cplx<T>& acc, const cplx<T>& v, const cplx<T>& w) -> cplx<T> {
+    if constexpr (sizeof(T) == 8)
+      return mk(acc.re + v.re * w.re + v.im * w.im, acc.im + v.im * w.re - v.re * w.im);
+    else
+      return acc + v * conj(w);
   };
 
 
@@ -493,7 +506,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       DFT<T, P, false>::run(v);
       if constexpr (HOIST) {
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) yrow[(k1 * Q + li) * SP] = v[k1] * t2v[HOIST ? k1 : 0];
+        for (int k1 = 0; k1 < P; ++k1) yrow[(k1 * Q + li) * SP] = k1 == 0 ? v[0] : v[k1] * t2v[HOIST ? k1 : 0];
       } else {
         mul_store(std::integral_constant<int, P>{}, v, [&](int k1) { return T2[li * P + k1]; },
                   [&](int k1, const cplx<T>& x) { yrow[(k1 * Q + li) * SP] = x; });
@@ -523,7 +536,6 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       for (int j1 = 0; j1 < P; ++j1) {
         const int z = j1 * Q + li;
         u[j1] = z < n ? ycol[z * SR] : mk((T)0, (T)0);
-        acc[j1] = mk((T)0, (T)0);
       }
     }
     // FP32 with P = Q: the nine T2 twiddles of this lane serve the forward and
@@ -551,7 +563,8 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         DFT<T, P, false>::run(v);
         if constexpr (HOIST) {
 #pragma unroll
-          for (int k1 = 0; k1 < P; ++k1) ycol[(k1 * Q + li) * SR] = v[k1] * (H2 ? t2v[H2 ? k1 : 0] : t2f(k1));
+          for (int k1 = 0; k1 < P; ++k1)
+            ycol[(k1 * Q + li) * SR] = k1 == 0 ? v[0] : v[k1] * (H2 ? t2v[H2 ? k1 : 0] : t2f(k1));
         } else {
           mul_store(std::integral_constant<int, P>{}, v, t2f,
                     [&](int k1, const cplx<T>& x) { ycol[(k1 * Q + li) * SR] = x; });
@@ -573,7 +586,8 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         DFT<T, Q, true>::run(w);
         if constexpr (HOIST) {
 #pragma unroll
-          for (int j2 = 0; j2 < Q; ++j2) ycol[(li * Q + j2) * SR] = w[j2] * conj(H2 ? t2v[H2 ? j2 : 0] : t2i(j2));
+          for (int j2 = 0; j2 < Q; ++j2)
+            ycol[(li * Q + j2) * SR] = j2 == 0 ? w[0] : w[j2] * conj(H2 ? t2v[H2 ? j2 : 0] : t2i(j2));
         } else {
           mul_store(std::integral_constant<int, Q>{}, w, [&](int j2) { return conj(t2i(j2)); },
                     [&](int j2, const cplx<T>& x) { ycol[(li * Q + j2) * SR] = x; });
@@ -587,7 +601,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         DFT<T, P, true>::run(v);
         const cplx<T> wb = odd_base();
 #pragma unroll
-        for (int j1 = 0; j1 < P; ++j1) acc[j1] = acc[j1] + (cz ? v[j1] * conj(odd_tw(wb, j1)) : v[j1]);
+        for (int j1 = 0; j1 < P; ++j1) acc[j1] = cz ? macc_conj(acc[j1], v[j1], odd_tw(wb, j1)) : v[j1];
       }
     };
     // both channels unrolled (cz known at compile time): with a runtime
@@ -603,8 +617,10 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       }
     }
   };
-  // y-inverse of row g for channel cy -> o[j1] at y = j1 Q + li
-  auto yinv = [&](int cy, int g, cplx<T>* o) {
+  // y-inverse of row g for channel cy -> o[j1] at y = j1 Q + li; the odd
+  // channel's output is added to add[j1] (the even channel's part) in the
+  // same fused multiply-adds that apply its w_M^-y
+  auto yinv = [&](int cy, int g, cplx<T>* o, const cplx<T>* add) {
     const bool rowok = g < n;
     cplx<T>* yrow = Y + g * SR;
     if (rowok && li < P) {
@@ -615,7 +631,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       DFT<T, Q, true>::run(u);
       if constexpr (HOIST) {
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) yrow[(li * Q + j2) * SP] = u[j2] * conj(t2v[HOIST ? j2 : 0]);
+        for (int j2 = 0; j2 < Q; ++j2) yrow[(li * Q + j2) * SP] = j2 == 0 ? u[0] : u[j2] * conj(t2v[HOIST ? j2 : 0]);
       } else {
         mul_store(std::integral_constant<int, Q>{}, u, [&](int j2) { return conj(T2[j2 * P + li]); },
                   [&](int j2, const cplx<T>& x) { yrow[(li * Q + j2) * SP] = x; });
@@ -629,7 +645,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       DFT<T, P, true>::run(v);
       const cplx<T> wb = odd_base();
 #pragma unroll
-      for (int j1 = 0; j1 < P; ++j1) o[j1] = cy ? v[j1] * conj(odd_tw(wb, j1)) : v[j1];
+      for (int j1 = 0; j1 < P; ++j1) o[j1] = cy ? macc_conj(add[j1], v[j1], odd_tw(wb, j1)) : v[j1];
     }
   };
 
@@ -648,11 +664,11 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   __syncthreads();
   const int g = grp;
   cplx<T> o[P], keep[P];
-  yinv(0, g, o);
+  yinv(0, g, o, o);
   load_row(base, g, x);  // the cy = 1 input: the same rows again, mostly from L2
   if (KEEP) {
 #pragma unroll
-    for (int j1 = 0; j1 < P; ++j1) keep[j1] = o[j1];  // cy = 0 output part, in registers (FP32)
+    for (int j1 = 0; j1 < P; ++j1) keep[j1] = o[j1];  // cy = 0 output part, in registers
     yfwd(1, g, x);
   } else {
     yfwd(1, g, x);  // consumes x before its HBM slots take the parked cy = 0 output
@@ -686,12 +702,12 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       }
     }
   }
-  yinv(1, g, o);
+  yinv(1, g, o, keep);  // o = keep + the odd channel's part
   if (g < n && li < Q) {
 #pragma unroll
     for (int j1 = 0; j1 < P; ++j1) {
       const int y = j1 * Q + li;
-      if (y < n) base[(size_t)g * n + y] = keep[j1] + o[j1];
+      if (y < n) base[(size_t)g * n + y] = o[j1];
     }
   }
 }
