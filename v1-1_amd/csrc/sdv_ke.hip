@@ -179,7 +179,7 @@ constexpr int kKeMaxQT = 6;                  // q tiles of 16: n <= 96
 // the plane (the z contraction follows in k_ke_z):
 //   rx(x, y) = post_x(x) sum_p K(x, p) pre_x(p) in(p, y)
 //   ry(x, y) = post_y(y) sum_p K(y, p) pre_y(p) in(x, p)
-//   out(x, y) = ((out or 0) + rx) + ry          (axis 0, then axis 1)
+//   out(x, y) = ((out or 0) + rx) + ry          (out: the z pass's result)
 // with K(q, p) = ketot(q - p), pre/post the ECS 1/J of the axis, the edge
 // mask divided out before and multiplied back after, and the axis
 // coefficient (SD/coreproject.f90:1321-1390).  Both terms are 16 x 16 output
@@ -231,7 +231,7 @@ static size_t ke_xy_lds_bytes(int n, int mode) {
 // BOTH: the x and y axes are both on (every kinetic apply; the velocity gauge
 // may switch one off)
 template <typename T, int LD, int MODE, bool BOTH>
-__global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const double* __restrict__ ke,
+__global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int nplane, const double* __restrict__ ke,
                                                          const z64* __restrict__ invj,
                                                          const double* __restrict__ mask,
                                                          const z64* __restrict__ in, z64* __restrict__ out, z64 cx,
@@ -269,19 +269,32 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const 
     for (int i = tid; i < n; i += kKeXYThreads) ms[i] = mask[i];
   for (int i = tid; i < (two ? 2 : 1) * NP * pitch; i += kKeXYThreads) plx[i] = mk((T)0, (T)0);
 
-  z64 pf[LD];  // the next plane, in flight during the current plane's MFMAs
+  // Tasks: the planes in full rounds of gridDim.x, then the R planes left
+  // over each split S ways (S = gridDim.x / R, at most one per tile), sub-task
+  // s taking the tiles and edge points congruent to s mod S: every workgroup
+  // gets the same number of tasks instead of R workgroups running a last
+  // round alone (81^3 with 16 orbitals: 1296 planes = 5 x 256 + 16).
+  const int G = gridDim.x, R = nplane % G, NT2 = NT * NT;
+  const int S = R ? max(1, min(G / R, NT2)) : 1, nfull = nplane - R, ntask = nfull + R * S;
+  auto plane_of = [&](int t) { return t < nfull ? t : nfull + (t - nfull) / S; };
+  // ECS (FP64: operands scaled as they are read) keeps no next plane in
+  // registers: it would spill there
+  constexpr bool PF = !ECS;
+  z64 pf[PF ? LD : 1];  // the next plane, in flight during the current plane's MFMAs
   auto prefetch = [&](int task) {
-    const z64* src = in + (size_t)(task / n) * n3 + (size_t)(task % n) * n2;
+    const int pl = plane_of(task);
+    const z64* src = in + (size_t)(pl / n) * n3 + (size_t)(pl % n) * n2;
 #pragma unroll
-    for (int k = 0; k < LD; ++k) {
+    for (int k = 0; k < (PF ? LD : 0); ++k) {
       const int e = tid + k * kKeXYThreads;
       if (e < ne) pf[k] = src[e];
     }
   };
   // out(x, y) of plane z of orbital o from the two contractions
-  auto emit = [&](int o, int z, int x, int y, z64 rx, z64 ry) {
+  // (prev: out(x, y) as read before the contractions, or 0)
+  auto emit = [&](int o, int z, int x, int y, z64 rx, z64 ry, z64 prev) {
     const size_t g = (size_t)o * n3 + (size_t)z * n2 + (size_t)y * n + x;
-    z64 r = (flags & kKeAccumulate) ? out[g] : mk(0.0, 0.0);
+    z64 r = prev;
     if (on_x) {
       if (ECS) rx = rx * ijx[x];                          // temp2 * invjacobian   :1366-1368
       if (MASK) rx = mask_multiply_xyz(rx, ms, x, y, z);  // mult_mask   :1373-1374
@@ -296,9 +309,11 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const 
     }
     out[g] = r;
   };
-  if ((int)blockIdx.x < ntask) prefetch(blockIdx.x);
-  for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
-    const int o = task / n, z = task - o * n;
+  if (PF && (int)blockIdx.x < ntask) prefetch(blockIdx.x);
+  for (int task = blockIdx.x; task < ntask; task += G) {
+    const int pl = plane_of(task), sub = task < nfull ? 0 : (task - nfull) % S, ss = task < nfull ? 1 : S;
+    const int o = pl / n, z = pl - o * n;
+    const z64* src = in + (size_t)o * n3 + (size_t)z * n2;
     __syncthreads();  // every wave is done with the previous plane (and the tables are stored)
     // element e = tid + k * threads sits at (y, x); stepped without divisions
     int y = tid / n, x = tid - (tid / n) * n;
@@ -307,7 +322,7 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const 
     for (int k = 0; k < LD; ++k) {
       const int e = tid + k * kKeXYThreads;
       if (e < ne) {
-        z64 v = pf[k];
+        z64 v = PF ? pf[PF ? k : 0] : src[e];
         if (MASK) v = mask_divide_xyz(v, ms, x, y, z);  // divide_mask   :1355-1356
         if (two) {  // temp = in * invjacobian(:,d), rounded once   :1344-1346
           plx[y * pitch + x] = cvt<T>(v * ijx[x]);
@@ -324,8 +339,8 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const 
       }
     }
     __syncthreads();
-    if (task + (int)gridDim.x < ntask) prefetch(task + gridDim.x);
-    for (int tile = w; tile < NT * NT; tile += kKeXYWaves) {
+    if (PF && task + G < ntask) prefetch(task + G);
+    for (int tile = sub + ss * w; tile < NT2; tile += ss * kKeXYWaves) {
       const int yt = tile / NT, xt = tile - yt * NT;
       // operand addresses at p0 = 0 (rows and columns beyond the plane read
       // a valid row: their outputs are discarded)
@@ -333,6 +348,15 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const 
       const cplx<T>* by = ply + k4 * pitch + min(16 * xt + i16, NP - 1);  // ry: B = plane(row p, col x), + p0 pitch
       const T* bx = sk + 16 * xt + i16 - k4 + OFF;                         // rx: B = K(x, p), - p0
       const T* ay = sk + 16 * yt + i16 - k4 + OFF;                         // ry: A = K(y, p), - p0
+      // the tile's values of out, read now and added after the contractions
+      const int xo = 16 * xt + i16;
+      z64 prev[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int y = 16 * yt + Mma<T>::row(l, v);
+        prev[v] = mk(0.0, 0.0);
+        if ((flags & kKeAccumulate) && y < n && xo < n) prev[v] = out[(size_t)o * n3 + (size_t)z * n2 + (size_t)y * n + xo];
+      }
       v4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0}, yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
       // two operand sets, each read a half step ahead of its MFMAs (no
       // register copies, so no wait is forced inside the half step)
@@ -375,17 +399,17 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const 
         }
         if (h1) step(da1, db1, sa1, sb1, p0 + 4);
       }
-      const int x = 16 * xt + i16;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int y = 16 * yt + Mma<T>::row(l, v);
-        if (y < n && x < n) emit(o, z, x, y, mk((double)xr[v], (double)xi[v]), mk((double)yr[v], (double)yi[v]));
+        if (y < n && xo < n) emit(o, z, xo, y, mk((double)xr[v], (double)xi[v]), mk((double)yr[v], (double)yi[v]), prev[v]);
       }
     }
     // the last row and column on the VALU, by the wavefronts that hold one
     // tile fewer (lane e: point (n-1, e) for e < n, else (e-n, n-1))
     if (edge) {
-      const int w0 = (NT * NT) % kKeXYWaves, e = (w - w0) * 64 + l;
+      const int nts = (NT2 - sub + ss - 1) / ss;  // tiles of this (sub-)task
+      const int w0 = nts < kKeXYWaves ? nts : nts % kKeXYWaves, e = sub + ss * ((w - w0) * 64 + l);
       if (w >= w0 && e < 2 * n - 1) {
         const int x = e < n ? n - 1 : e - n, y = e < n ? e : n - 1;
         double xre = 0.0, xim = 0.0, yre = 0.0, yim = 0.0;
@@ -402,14 +426,15 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const 
           yre = fma((double)b.re, (double)ky, yre);
           yim = fma((double)b.im, (double)ky, yim);
         }
-        emit(o, z, x, y, mk(xre, xim), mk(yre, yim));
+        const z64 prev = (flags & kKeAccumulate) ? out[(size_t)o * n3 + (size_t)z * n2 + (size_t)y * n + x] : mk(0.0, 0.0);
+        emit(o, z, x, y, mk(xre, xim), mk(yre, yim), prev);
       }
     }
   }
 }
 // ---------------------------------------------------------------- z pass
-// The z contraction added into `out` (SD/coreproject.f90:1321-1390):
-//   out(x, y, q) += post_z(q) sum_p K(q, p) pre_z(p) in(x, y, p)  [+ scalediag in]
+// The z contraction, the first of the two passes (SD/coreproject.f90:1321-1390):
+//   out(x, y, q) = (out or 0) + post_z(q) sum_p K(q, p) pre_z(p) in(x, y, p)  [+ scalediag in]
 // from registers: one wavefront per 16 consecutive (x, y) columns of one
 // orbital.  Lane l holds the B operand of every 4-step s of the contraction,
 // column 16 cb + l % 16 at z = 4 s + l / 16 (for one s the wavefront reads
@@ -417,11 +442,16 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int ntask, const 
 // is the A operand, read from a Toeplitz table in LDS; each 16-row q tile is
 // added into `out` straight from its accumulators, 16 consecutive columns per
 // row.  The data never passes through LDS and the wavefronts share no
-// barrier, so a CU's wavefronts overlap one another's loads and MFMAs.
+// barrier, so a CU's wavefronts overlap one another's loads and MFMAs.  It
+// runs first because it is the memory-bound pass: writing `out` without
+// reading it leaves the read-modify-write of `out` to the MFMA-bound x/y
+// pass, which prefetches it (81^3: z 108 -> <z> us, x/y <xy> us).
 constexpr int kKeZWaves = 4;
 
+// three wavefronts per SIMD (at most 168 registers): the overlap of one
+// wavefront's loads with another's MFMAs is what this pass runs on
 template <typename T, int NS, int MODE>
-__global__ void __launch_bounds__(64 * kKeZWaves) k_ke_z(int n, int ntask, const double* __restrict__ ke,
+__global__ void __launch_bounds__(64 * kKeZWaves) __attribute__((amdgpu_waves_per_eu(3))) k_ke_z(int n, int ntask, const double* __restrict__ ke,
                                                           const z64* __restrict__ invj,
                                                           const z64* __restrict__ sdiag,
                                                           const double* __restrict__ mask,
@@ -459,8 +489,26 @@ __global__ void __launch_bounds__(64 * kKeZWaves) k_ke_z(int n, int ntask, const
       t[s] = cvt<T>(v);
     }
   }
+  // out(x, y, q) from the contraction r
+  auto put = [&](int q, z64 r) {
+    const size_t g = base + (size_t)q * n2;
+    if (ECS) r = r * ijz[q];                              // temp2 * invjacobian   :1366-1368
+    if (MASK) r = mask_multiply_xyz(r, mask, xc, yc, q);  // mult_mask   :1373-1374
+    if (flags & kKeCoef) r = r * cz;                      // * mycoefs   :1368,1374
+    z64 acc = (flags & kKeAccumulate) ? out[g] + r : mk(0.0, 0.0) + r;
+    if (sdiag) {  // out + in * scalediag   :1382-1388
+      const z64 sd = ((mk(0.0, 0.0) + sdiag[xc]) + sdiag[n + yc]) + sdiag[2 * n + q];
+      acc = acc + in[g] * sd;
+    }
+    out[g] = acc;
+  };
+  // with n = 16 k + 1 the last row q = n - 1 is a dot product on the VALU
+  // (each lane's share of p, summed over the column's four lanes) instead of
+  // a seventeenth-row tile
+  const bool edge = n % 16 == 1;
+  const int QTM = edge ? n / 16 : QT;
   const T* arow = sk + i16 - k4 + OFF;  // A(q, p) = sk[16 qt + i16 - (4 s + k4) + OFF]
-  for (int qt = 0; qt < QT; ++qt) {
+  for (int qt = 0; qt < QTM; ++qt) {
     v4 ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -473,20 +521,25 @@ __global__ void __launch_bounds__(64 * kKeZWaves) k_ke_z(int n, int ntask, const
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int q = 16 * qt + Mma<T>::row(l, v);
-      if (q < n && colok) {
-        const size_t g = base + (size_t)q * n2;
-        z64 r = mk((double)ar[v], (double)ai[v]);
-        if (ECS) r = r * ijz[q];                              // temp2 * invjacobian   :1366-1368
-        if (MASK) r = mask_multiply_xyz(r, mask, xc, yc, q);  // mult_mask   :1373-1374
-        if (flags & kKeCoef) r = r * cz;                      // * mycoefs   :1368,1374
-        z64 acc = out[g] + r;
-        if (sdiag) {  // out + in * scalediag   :1382-1388
-          const z64 sd = ((mk(0.0, 0.0) + sdiag[xc]) + sdiag[n + yc]) + sdiag[2 * n + q];
-          acc = acc + in[g] * sd;
-        }
-        out[g] = acc;
+      if (q < n && colok) put(q, mk((double)ar[v], (double)ai[v]));
+    }
+  }
+  if (edge) {
+    double re = 0.0, im = 0.0;
+    const T* krow = sk + (n - 1) - k4 + OFF;  // K(n - 1, 4 s + k4)
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (s < ns) {
+        const double k = (double)krow[-4 * s];
+        re = fma((double)t[s].re, k, re);
+        im = fma((double)t[s].im, k, im);
       }
     }
+    re += __shfl_xor(re, 16);
+    im += __shfl_xor(im, 16);
+    re += __shfl_xor(re, 32);
+    im += __shfl_xor(im, 32);
+    if (k4 == 0 && colok) put(n - 1, mk(re, im));
   }
 }
 
@@ -496,7 +549,7 @@ static int ke_z_launch1(int n, int howmany, hipStream_t st, const KeArgsHost& a)
   const size_t sh = (((size_t)ke_xy_sk(n) * sizeof(T) + 15) & ~(size_t)15);
   hipLaunchKernelGGL((k_ke_z<T, NS, MODE>), dim3((ntask + kKeZWaves - 1) / kKeZWaves), dim3(64 * kKeZWaves), sh, st,
                      n, ntask, a.ke, (const z64*)a.invj, (const z64*)a.sdiag, a.mask, (const z64*)a.in, (z64*)a.out,
-                     a.coef[2], (a.use_coef ? kKeCoef : 0));
+                     a.coef[2], (a.use_coef ? kKeCoef : 0) | (a.accumulate ? kKeAccumulate : 0));
   return SDV_OK;
 }
 template <typename T>
@@ -565,21 +618,21 @@ static int ke_launch_mma(const KeArgsHost& a, hipStream_t st) {
   SDV_REQUIRE(n <= 16 * kKeMaxQT, SDV_ERR_UNSUPPORTED, "grid edge too large for the MFMA kinetic tile");
   const z64* ij = (const z64*)a.invj;
   const int flags = (a.use_coef ? kKeCoef : 0) | (a.accumulate ? kKeAccumulate : 0);
-  const bool on0 = axis_on(a, 0), on1 = axis_on(a, 1);
-  if (on0 || on1) {  // x and y together: k_ke_xy
+  const bool on0 = axis_on(a, 0), on1 = axis_on(a, 1), on2 = axis_on(a, 2);
+  if (on2) {  // z first: out = (out or 0) + z term (+ scalediag in)
+    const int rc = ke_z_launch<T>(n, st, a);
+    if (rc != SDV_OK) return rc;
+  }
+  if (on0 || on1) {  // x and y together: k_ke_xy, adding into the z pass's out
     const int ld = (n * n + kKeXYThreads - 1) / kKeXYThreads;
     const int mode = ij ? kKeEcs : a.mask ? kKeMask : kKePlain;
     const size_t shxy = ke_xy_lds_bytes<T>(n, mode);
     SDV_REQUIRE(ld <= 12 && shxy <= 160 * 1024, SDV_ERR_UNSUPPORTED, "grid edge too large for the x/y kinetic pass");
     const int ntask = a.howmany * n;
     const int grid = std::min(ntask, cu_count());
-    const int fl = flags | (on0 ? kKeXOn : 0) | (on1 ? kKeYOn : 0);
+    const int fl = flags | (on0 ? kKeXOn : 0) | (on1 ? kKeYOn : 0) | (on2 ? kKeAccumulate : 0);
     const int rc = ld <= 9 ? ke_xy_launch<T, 9>(mode, grid, shxy, st, n, ntask, a, ij, fl)
                            : ke_xy_launch<T, 12>(mode, grid, shxy, st, n, ntask, a, ij, fl);
-    if (rc != SDV_OK) return rc;
-  }
-  if (axis_on(a, 2)) {
-    const int rc = ke_z_launch<T>(n, st, a);
     if (rc != SDV_OK) return rc;
   }
   SDV_HIP_CHECK(hipGetLastError());
