@@ -325,12 +325,20 @@ def main():
         tk = time_steps(lambda: s.mult_ke(phi), args.steps, 1, torch)
         ke_out[name] = s.mult_ke(phi)
         useful = 12.0 * n ** 4 * N                                    # 3 axes x 2 (re, im) x 2 n^4
-        npad, qpad = (n + 3) // 4 * 4, (n + 15) // 16 * 16
-        issued = 3 * 2 * 2.0 * qpad * npad * n * n * N               # padded matrix-core work
+        # matrix-core work as issued: the x/y pass runs nt^2 16 x 16 tiles per
+        # plane (2 terms, re and im, np/4 steps of 16x16x4), the z pass nq
+        # 16-row tiles per 16 columns; with n = 16k + 1 the last row/column is
+        # on the VALU and not counted
+        npad = (n + 3) // 4 * 4
+        nt = n // 16 if n % 16 == 1 else (n + 15) // 16
+        issued_xy = nt * nt * 2 * 2 * (npad // 4) * 2048.0 * n * N
+        issued_z = ((n * n + 15) // 16) * nt * 2 * (npad // 4) * 2048.0 * N
+        issued = issued_xy + issued_z
+        hbm = 5 * 16.0 * n ** 3 * N   # z pass: in read, out written; x/y pass: in read, out read and written
         ke_res[name] = {"ms_per_call_16_orbitals": tk * 1e3, "orbitals_per_s": N / tk,
                         "mfma_tflops_useful": useful / tk / 1e12, "mfma_tflops_issued": issued / tk / 1e12,
                         "mfma_peak_tflops": peak, "mfma_utilisation": issued / tk / 1e12 / peak,
-                        "hbm_frac_min_traffic": 8 * 16.0 * n ** 3 * N / tk / 1e9 / HBM_PEAK_GBS}
+                        "hbm_bytes_per_call": hbm, "hbm_frac": hbm / tk / 1e9 / HBM_PEAK_GBS}
     s.set_precision(0)
     ke_res["mixed"]["error_vs_fp64_max_abs_over_max"] = float(
         (ke_out["mixed"] - ke_out["fp64"]).abs().max() / ke_out["fp64"].abs().max())
@@ -373,7 +381,8 @@ def main():
             traffic = traffic * conv_per_launch / tj["convolutions_per_launch"]
         else:
             traffic = None                            # counters of another work split
-        counters = {k: tj[k] for k in ("valu_busy", "lds_bank_conflict_frac", "waves_per_cu", "source") if k in tj}
+        counters = {k: tj[k] for k in ("valu_busy", "valu_busy_source", "lds_bank_conflict_frac", "wait_any_frac",
+                                       "waves_per_cu", "source") if k in tj}
     out = {
         "metric": METRIC,
         "value": npairs / t64,
@@ -397,8 +406,9 @@ def main():
                      "traffic_frac": (traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
                      "alg_bytes_per_launch": alg, "alg_bytes": "fused kernel's floor: read the x-forward "
                      "output once, write the x-inverse input once (+ spectrum octant)",
-                     "limiter": "VALU issue and LDS exchanges (rocprofv3 SQ counters, profiles/)" if dom == "yz"
-                     else "HBM", "sq_counters": counters or None,
+                     "limiter": "FP64 VALU issue, with the LDS exchanges' writes overlapping it only in part "
+                     "(rocprofv3 VALUBusy and SQ counters, profiles/)" if dom == "yz" else "HBM",
+                     "sq_counters": counters or None,
                      "pairs_per_launch": pairs_per_launch,
                      "convolutions_per_launch": conv_per_launch, "avg_launch_ms": avg_ms},
         "path_effective_pencil_frac": alg_bytes_per_pair("path", n, M, 16) * nconv / t64 / 1e9 / HBM_PEAK_GBS,
@@ -415,8 +425,10 @@ def main():
         "twoematel_ms_fp64": tmat * 1e3,
         "mean_field_fp64": mean_field,
         "spectrum_setup_ms": t_setup * 1e3,
-        "mult_ke": dict(ke_res, note="per-axis contraction on the matrix cores (FP64 v_mfma_f64_16x16x4f64, "
-                                     "mixed v_mfma_f32_16x16x4f32); utilisation = issued matrix flops / peak"),
+        "mult_ke": dict(ke_res, note="two passes on the matrix cores (FP64 v_mfma_f64_16x16x4f64, mixed "
+                                     "v_mfma_f32_16x16x4f32): z contraction from registers, then the x and y "
+                                     "contractions of each z plane from one staged copy; utilisation = issued "
+                                     "matrix flops / peak"),
     }
     del tm64, tm32, tm, tr
     torch.cuda.empty_cache()

@@ -445,7 +445,8 @@ __global__ void __launch_bounds__(kKeXYThreads) k_ke_xy(int n, int nplane, const
 // barrier, so a CU's wavefronts overlap one another's loads and MFMAs.  It
 // runs first because it is the memory-bound pass: writing `out` without
 // reading it leaves the read-modify-write of `out` to the MFMA-bound x/y
-// pass, which prefetches it (81^3: z 108 -> <z> us, x/y <xy> us).
+// pass, which reads it ahead of each tile (81^3 FP64, 16 orbitals: z pass
+// 108 -> 90 us, whole apply 0.27 -> 0.25 ms).
 constexpr int kKeZWaves = 4;
 
 // three wavefronts per SIMD (at most 168 registers): the overlap of one

@@ -34,7 +34,26 @@ if os.path.dirname(HERE) not in sys.path:
 COPY_BYTES = 1 << 30          # read and written once by the calibration copy
 PAIRS_PER_LAUNCH = 136        # every pair of the 16-orbital step in one launch (the default)
 CONV_PER_LAUNCH = 128         # its convolutions: 120 off-diagonal pairs + 16 diagonal ones two per convolution
-KERNELS = ("k_xfwd", "k_yz", "k_xinv")
+KERNELS = ("k_xfwd", "k_yz", "k_xinv", "k_ke_xy", "k_ke_z")
+
+
+def run_ke(precision, steps):
+    """The kinetic apply (mult_ke) of the 81^3 / 16-orbital orbitals, plain and
+    with ECS (the 41^3 workload's ramp parameters)."""
+    import numpy as np
+    import torch
+
+    from sincdvr import SincDVR, inputs
+    cfg = inputs.CONFIGS["grid81_16orb"]
+    n, dx, N = cfg["n"], cfg["spacing"], cfg["norb"]
+    phi = torch.from_numpy(np.ascontiguousarray(
+        inputs.gaussian_orbitals(n, dx, N, cfg["ngauss"], cfg["seed"]).T)).cuda()
+    for sc in (None, inputs.ECS_41):
+        s = SincDVR(n, dx, N, scaling=sc, precision=precision)
+        for i in range(steps):
+            s.mult_ke(phi)
+        torch.cuda.synchronize()
+        print(f"[profiling] mult_ke ecs={sc is not None} done", flush=True)
 
 
 def run(precision, steps):
@@ -222,12 +241,60 @@ def summarize_sq(out_dir, prof_dir):
     return lines
 
 
+def summarize_util(out_dir, prof_dir):
+    """VALUBusy and MfmaUtil (rocprofv3's derived counters: VALU active
+    quad-cycles, and matrix-core busy cycles, over the dispatch's GPU-active
+    cycles) of the convolution kernels (OUT/util0, OUT/util1: one pass per
+    precision over the pair-potential workload) and of the kinetic kernels
+    (OUT/ke0, OUT/ke1: the kinetic workload); writes
+    profiles/utilisation_81_n16.md and adds the FP64 y/z kernel's VALUBusy to
+    profiles/traffic_k_yz_81.json."""
+    lines = ["# VALUBusy / MfmaUtil at 81^3, 16 orbitals", "",
+             "Commands (repository root, on the GPU; P = 0 FP64, 1 mixed):", "",
+             "    rocprofv3 --pmc VALUBusy MfmaUtil --output-format csv -d OUT/utilP -o run -- python3 "
+             "v1-1_amd/sincdvr/profiling.py run --precision P --steps 1",
+             "    rocprofv3 --pmc VALUBusy MfmaUtil --output-format csv -d OUT/keP -o run -- python3 "
+             "v1-1_amd/sincdvr/profiling.py run --what ke --precision P --steps 3",
+             "    python3 v1-1_amd/sincdvr/profiling.py util OUT profiles", "",
+             "Per dispatch (the largest grid of each kernel; kinetic kernels: the mean over the plain and",
+             "ECS dispatches of the workload).", "",
+             "| kernel | VALUBusy % | MfmaUtil % |", "|---|---:|---:|"]
+    yz64 = None
+    # each pass lists the kernels of its own element type and family (the
+    # mixed passes also run FP64 convolutions, for the charge carrier)
+    for sub, elem, fam in (("util0", "<double,", "k_x"), ("util0", "<double,", "k_yz"),
+                           ("util1", "<float,", "k_x"), ("util1", "<float,", "k_yz"),
+                           ("ke0", "<double,", "k_ke"), ("ke1", "<float,", "k_ke")):
+        rows = _rows(os.path.join(out_dir, sub, "**", "*counter_collection.csv"))
+        vb = counter_per_dispatch(rows, "VALUBusy")
+        mu = counter_per_dispatch(rows, "MfmaUtil")
+        for k in sorted(vb):
+            if k == "copy" or elem not in k or ("sdv::" + fam) not in k:
+                continue
+            lines.append("| %s | %.1f | %.1f |" % (k, vb[k], mu.get(k, 0.0)))
+            if sub == "util0" and k.startswith("sdv::k_yz<double"):
+                yz64 = vb[k] / 100.0
+    os.makedirs(prof_dir, exist_ok=True)
+    with open(os.path.join(prof_dir, "utilisation_81_n16.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    tj = os.path.join(prof_dir, "traffic_k_yz_81.json")
+    if yz64 is not None and os.path.exists(tj):
+        with open(tj) as f:
+            t = json.load(f)
+        t["valu_busy"] = yz64
+        t["valu_busy_source"] = "rocprofv3 VALUBusy (profiles/utilisation_81_n16.md)"
+        with open(tj, "w") as f:
+            json.dump(t, f, indent=1)
+    return lines
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     r = sub.add_parser("run")
     r.add_argument("--precision", type=int, default=0)
     r.add_argument("--steps", type=int, default=3)
+    r.add_argument("--what", choices=("pairpot", "ke"), default="pairpot")
     sm = sub.add_parser("summarize")
     sm.add_argument("out_dir")
     sm.add_argument("prof_dir")
@@ -235,11 +302,16 @@ def main():
     sq = sub.add_parser("sq")
     sq.add_argument("out_dir")
     sq.add_argument("prof_dir")
+    ut = sub.add_parser("util")
+    ut.add_argument("out_dir")
+    ut.add_argument("prof_dir")
     a = ap.parse_args()
     if a.cmd == "run":
-        run(a.precision, a.steps)
+        (run_ke if a.what == "ke" else run)(a.precision, a.steps)
     elif a.cmd == "sq":
         print("\n".join(summarize_sq(a.out_dir, a.prof_dir)))
+    elif a.cmd == "util":
+        print("\n".join(summarize_util(a.out_dir, a.prof_dir)))
     else:
         print("\n".join(summarize(a.out_dir, a.prof_dir, a.tag)))
 
