@@ -169,8 +169,15 @@ struct XCfg {
 #define SDV_YZ_TWB 3
 #endif
 
+// Plane pitch in elements.  FP64 (16-byte elements): the smallest p >= R with
+// p = 9 mod 16, which together with the exchange slot maps of body_yz makes
+// every LDS read of the plane free of bank conflicts (64 banks of 4 bytes, a
+// b128 access served 16 lanes at a time: at p = 81 half of them are two-way
+// conflicted, 8.2 against 4.9 cycles per wave instruction, while writes cost
+// the same either way).  FP32: R | 1.
+constexpr int yz_pitch(int R, int es) { return es == 16 ? R + ((9 - R) & 15) : (R | 1); }
 constexpr size_t yz_base_bytes(int R, int es) {
-  return (size_t)(3 * R + (R & 1) + R * (R | 1)) * es;  // tables (2R + R + pad) and plane, bytes
+  return (size_t)(3 * R + (R & 1) + R * yz_pitch(R, es)) * es;  // tables (2R + R + pad) and plane, bytes
 }
 constexpr bool yz_khat_in_lds(int R, int es) {
   return es == 8 && SDV_YZ_KHAT_LDS && yz_base_bytes(R, es) + (size_t)(R + 1) * (R + 1) * (es / 2) <= 160 * 1024;
@@ -344,8 +351,8 @@ __global__ void __launch_bounds__(XThreads<T>::fwd) k_xfwd(KArgs<T> a) {
 //            ky contiguous: the column FFTs, which hold the most registers,
 //            address it from one base with immediate offsets), [z][ky] in FP32
 // Per y channel cy (even, odd):
-//   rows    : y-forward of row z in place (input y < n from HBM), positions
-//             k1*Q + k2 hold frequency k1 + P*k2;
+//   rows    : y-forward of row z in place (input y < n from HBM), position
+//             k1 + P*k2 (FP64) or k1*Q + k2 (FP32) holds frequency k1 + P*k2;
 //   columns : for the column of one ky, both z channels one after the other
 //             by the same lane group (its input kept in registers), each a
 //             forward FFT, the multiply by the real even spectrum and the
@@ -359,8 +366,9 @@ __global__ void __launch_bounds__(XThreads<T>::fwd) k_xfwd(KArgs<T> a) {
 // rows z < yz_park_rows in LDS after the plane, each lane in
 // slots of its own, the other rows in the plane's own HBM slots, which the
 // same lanes have just read for the cy = 1 input; the parked values are loaded
-// before the last y-inverse, so their latency overlaps it.  Row pitches other
-// than R | 1 were measured no faster (even pitches 18 % slower).
+// before the last y-inverse, so their latency overlaps it.  The pitch is
+// yz_pitch (FP32: other row pitches than R | 1 measured no faster, even ones
+// 18 % slower).
 // The LDS offsets are runtime values, so the compiler cannot prove the
 // twiddle tables distinct from the plane and does not pin hoisted twiddles
 // in VGPRs across phases.
@@ -383,7 +391,7 @@ __device__ __forceinline__ void wave_sync() {
 // time.
 template <typename T, int P, int Q, bool GUARD, typename V, bool FULL = false>
 __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, unsigned char* smem_raw) {
-  constexpr int R = P * Q, M = 2 * R, pitch = R | 1;
+  constexpr int R = P * Q, M = 2 * R, pitch = yz_pitch(R, (int)sizeof(cplx<T>));
   typedef YZCfg<T, P, Q> C;
   constexpr int G = C::G, GPW = C::GPW, NT = C::NT;
   // the cy = 0 output part stays in registers in FP32 and in FP64 on full
@@ -394,6 +402,15 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   // column contiguous (its column FFTs are the register-bound phase), FP32
   // keeps a row contiguous
   constexpr int SP = sizeof(T) == 8 ? pitch : 1, SR = sizeof(T) == 8 ? 1 : pitch;
+  // Slots of the forward exchanges and of the y-forward output.  FP64 puts the
+  // value of input index j2 and output index k1 of a line FFT at slot
+  // j2 P + k1 (written with the writer lane's j2 in the high digit, read with
+  // the reader lane's k1 in the low one) and frequency k1 + P k2 at position
+  // k1 + P k2, so that with its pitch every read has consecutive slots across
+  // a lane group; FP32 uses slot k1 Q + j2 and position k1 Q + k2.
+  constexpr bool XR = sizeof(T) == 8;
+  auto xslot = [](int j2, int k1) { return XR ? j2 * P + k1 : k1 * Q + j2; };
+  auto fpos = [](int k1, int k2) { return XR ? k1 + P * k2 : k1 * Q + k2; };
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* T2 = tw + a.lds_tw2;
   cplx<T>* Y = tw + a.lds_y;
@@ -506,20 +523,20 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
       DFT<T, P, false>::run(v);
       if constexpr (HOIST) {
 #pragma unroll
-        for (int k1 = 0; k1 < P; ++k1) yrow[(k1 * Q + li) * SP] = k1 == 0 ? v[0] : v[k1] * t2v[HOIST ? k1 : 0];
+        for (int k1 = 0; k1 < P; ++k1) yrow[xslot(li, k1) * SP] = k1 == 0 ? v[0] : v[k1] * t2v[HOIST ? k1 : 0];
       } else {
         mul_store(std::integral_constant<int, P>{}, v, [&](int k1) { return T2[li * P + k1]; },
-                  [&](int k1, const cplx<T>& x) { yrow[(k1 * Q + li) * SP] = x; });
+                  [&](int k1, const cplx<T>& x) { yrow[xslot(li, k1) * SP] = x; });
       }
     }
     wave_sync();
     if (rowok && li < P) {
       cplx<T> u[Q];
 #pragma unroll
-      for (int j2 = 0; j2 < Q; ++j2) u[j2] = yrow[(li * Q + j2) * SP];
+      for (int j2 = 0; j2 < Q; ++j2) u[j2] = yrow[xslot(j2, li) * SP];
       DFT<T, Q, false>::run(u);
 #pragma unroll
-      for (int k2 = 0; k2 < Q; ++k2) yrow[(li * Q + k2) * SP] = u[k2];
+      for (int k2 = 0; k2 < Q; ++k2) yrow[fpos(li, k2) * SP] = u[k2];
     }
   };
   // both z channels of column c (ky position), spectrum of y channel cy
@@ -527,8 +544,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     if (c >= R) return;
     const T* kh = KH_LDS ? khs : khg;
     cplx<T>* ycol = Y + c * SP;  // column c: row z at ycol[z * SR]
-    const int k1y = c / Q, k2y = c - (c / Q) * Q;
-    const int ky = 2 * (k1y + P * k2y) + cy;
+    const int ky = XR ? 2 * c + cy : 2 * (c / Q + P * (c - (c / Q) * Q)) + cy;
     const int kcol = (R + 1) * min(ky, M - ky);  // this column's z frequencies, adjacent; 32-bit offset from a uniform base
     cplx<T> u[P], acc[P];
     if (li < Q) {
@@ -564,17 +580,17 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
         if constexpr (HOIST) {
 #pragma unroll
           for (int k1 = 0; k1 < P; ++k1)
-            ycol[(k1 * Q + li) * SR] = k1 == 0 ? v[0] : v[k1] * (H2 ? t2v[H2 ? k1 : 0] : t2f(k1));
+            ycol[xslot(li, k1) * SR] = k1 == 0 ? v[0] : v[k1] * (H2 ? t2v[H2 ? k1 : 0] : t2f(k1));
         } else {
           mul_store(std::integral_constant<int, P>{}, v, t2f,
-                    [&](int k1, const cplx<T>& x) { ycol[(k1 * Q + li) * SR] = x; });
+                    [&](int k1, const cplx<T>& x) { ycol[xslot(li, k1) * SR] = x; });
         }
       }
       wave_sync();
       if (li < P) {  // step 2, spectral multiply, inverse step 1
         cplx<T> w[Q];
 #pragma unroll
-        for (int j2 = 0; j2 < Q; ++j2) w[j2] = ycol[(li * Q + j2) * SR];
+        for (int j2 = 0; j2 < Q; ++j2) w[j2] = ycol[xslot(j2, li) * SR];
         cplx<T> t2v[H2 ? Q : 1];
         if constexpr (H2) t2_load(t2v, true);
         DFT<T, Q, false>::run(w);
@@ -626,7 +642,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     if (rowok && li < P) {
       cplx<T> u[Q], t2v[HOIST ? Q : 1];
 #pragma unroll
-      for (int k2 = 0; k2 < Q; ++k2) u[k2] = yrow[(li * Q + k2) * SP];
+      for (int k2 = 0; k2 < Q; ++k2) u[k2] = yrow[fpos(li, k2) * SP];
       if constexpr (HOIST) t2_load(t2v, true);
       DFT<T, Q, true>::run(u);
       if constexpr (HOIST) {
@@ -856,7 +872,7 @@ static int launch_pq(const Spectrum& s, const ConvArgsHost& h, hipStream_t st, P
   a.lds_tw2 = M;
   a.lds_y = M + R + ((M + R) & 1);  // keep the workspaces 16-byte aligned for FP32
   const size_t shB = yz_lds_bytes(R, (int)sizeof(cplx<T>));
-  SDV_REQUIRE(shB <= 160 * 1024 && shB >= (size_t)(a.lds_y + R * pitch) * sizeof(cplx<T>), SDV_ERR_UNSUPPORTED,
+  SDV_REQUIRE(shB <= 160 * 1024 && shB >= (size_t)(a.lds_y + R * yz_pitch(R, (int)sizeof(cplx<T>))) * sizeof(cplx<T>), SDV_ERR_UNSUPPORTED,
               "y/z plane does not fit in LDS");
   const bool full = h.n == R;
   const void* kyz = full ? (const void*)k_yz<T, P, Q, true> : (const void*)k_yz<T, P, Q, false>;
