@@ -87,6 +87,18 @@ struct ConvArgsHost {
 // lines = 7 cache lines at m = 81) do not straddle an extra cache line.
 __host__ __device__ inline int plane_stride(int n) { return (n * n + 15) & ~15; }
 
+// Compute units of the current device (cached per device; 256 if unknown).
+inline int device_cu_count() {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cache[dev]) {
+    int c = 0;
+    cache[dev] = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0 ? c : 256;
+  }
+  return cache[dev];
+}
+
 // Convolution pipeline: x-forward (pair density fused), y/z plane kernel with
 // the spectral multiply, x-inverse with output scatter.  precision 0 = FP64,
 // 1 = FP32 butterflies.

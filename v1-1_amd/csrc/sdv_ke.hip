@@ -573,16 +573,7 @@ static int ke_z_launch(int n, hipStream_t st, const KeArgsHost& a) {
 
 // compute units of the current device (the persistent x/y pass takes one
 // workgroup per CU), cached per device
-static int cu_count() {
-  static int cache[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cache[dev]) {
-    int c = 0;
-    cache[dev] = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0 ? c : 256;
-  }
-  return cache[dev];
-}
+static int cu_count() { return device_cu_count(); }
 
 // An axis with a zero coefficient is skipped, as the reference skips it
 // (SD/coreproject.f90:1339); only when the axes accumulate into `out`.
