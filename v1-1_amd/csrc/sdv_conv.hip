@@ -1068,7 +1068,9 @@ __global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz_grp(const BSetDev<T
                                                                const BItemDev* __restrict__ items,
                                                                const BWork* __restrict__ work, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const BWork wk = work[blockIdx.x];
+  // the group's list is plane-major with pairs fastest; each XCD takes one
+  // range of it, as in the single-set y/z kernel
+  const BWork wk = work[xcd_span(blockIdx.x, gridDim.x)];
   const BItemDev it = items[wk.item];
   const ItemView<T> v = batch_view(sets[it.set], it, X);
   body_yz<T, P, Q, false, ItemView<T>, FULL>(v, v.X + (size_t)wk.sub * v.ld, wk.sub, smem_raw);
@@ -1142,9 +1144,12 @@ BatchPlan batch_plan(const std::vector<BatchSetHost>& sets, int precision, int m
   }
   // FUSED: largest cost first so the round-robin of the persistent grid stays
   // balanced; GROUPED: by size (largest first), each size contiguous.
+  // planes: GROUPED plane-major within a size (pairs fastest), so the
+  // workgroups of one XCD share the spectrum slices of a few planes
   auto order = [mode](const Cost& a, const Cost& b) {
     if (mode == BATCH_GROUPED && a.key != b.key) return a.key > b.key;
     if (a.c != b.c) return a.c > b.c;
+    if (mode == BATCH_GROUPED && a.w.sub != b.w.sub) return a.w.sub < b.w.sub;
     if (a.w.item != b.w.item) return a.w.item < b.w.item;
     return a.w.sub < b.w.sub;
   };
