@@ -358,9 +358,13 @@ def main():
                            torch.randn((N * N, N * N), dtype=torch.float64, device="cuda", generator=gen))
     tmf = time_steps(lambda: s.get_reducedpot(tr, twoden), args.steps, 1, torch)
     mf_flops = 8.0 * n ** 3 * N ** 4
+    mf_issued = 6.0 * n ** 3 * N ** 4           # three real products per complex one on the matrix cores
     mean_field = {"ms_per_call": tmf * 1e3, "tflops": mf_flops / tmf / 1e12, "peak_tflops_f64": MFMA_PEAK_F64,
                   "frac": mf_flops / tmf / 1e12 / MFMA_PEAK_F64,
-                  "shape": "(n^3 x N^2) . (N^2 x N^2) complex, 8 n^3 N^4 real flops"}
+                  "mfma_tflops_issued": mf_issued / tmf / 1e12,
+                  "mfma_utilisation": mf_issued / tmf / 1e12 / MFMA_PEAK_F64,
+                  "shape": "(n^3 x N^2) . (N^2 x N^2) complex, 8 n^3 N^4 real flops (tflops, frac); issued as "
+                           "three real products per complex one (Gauss), 6 n^3 N^4 on the matrix cores"}
     del twoden
 
     # roofline of the dominant kernel of the FP64 form

@@ -45,20 +45,27 @@ __global__ void k_transpose_z64(int K, const z64* __restrict__ g, z64* __restric
   gt[i] = g[(size_t)k + (size_t)K * c];
 }
 
+#ifndef SDV_RP_3M
+#define SDV_RP_3M 1
+#endif
 __global__ void __launch_bounds__(kRmThreads) k_reducedpot_mfma(int M, int K, const z64* __restrict__ A,
                                                                const z64* __restrict__ Bt, z64* __restrict__ C) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int lr = l & 15, lk = l >> 4;
   const long long x0 = (long long)blockIdx.x * kRmRows + 32 * (w & 1);
   const int c0 = blockIdx.y * kRmCols + 64 * (w >> 1);
-  f64x4 are[2][4], aim[2][4];
+  // Three real products per complex one (Gauss): P1 = Re A Re B, P2 = Im A Im B,
+  // P3 = (Re A + Im A)(Re B + Im B); Re C = P1 - P2, Im C = P3 - P1 - P2.
+  // With SDV_RP_3M=0, four: Re C = Re A Re B - Im A Im B, Im C = Re A Im B +
+  // Im A Re B.
+  constexpr int NP = SDV_RP_3M ? 3 : 2;
+  f64x4 acc[NP][2][4];
 #pragma unroll
-  for (int rt = 0; rt < 2; ++rt)
+  for (int p = 0; p < NP; ++p)
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-      are[rt][ct] = f64x4{0, 0, 0, 0};
-      aim[rt][ct] = f64x4{0, 0, 0, 0};
-    }
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) acc[p][rt][ct] = f64x4{0, 0, 0, 0};
 #pragma unroll 2
   for (int k0 = 0; k0 < K; k0 += 4) {
     const int k = k0 + lk;
@@ -73,15 +80,31 @@ __global__ void __launch_bounds__(kRmThreads) k_reducedpot_mfma(int M, int K, co
       const int c = c0 + 16 * ct + lr;
       b[ct] = (c < K && k < K) ? Bt[(long long)k * K + c] : mk(0.0, 0.0);
     }
+    if (SDV_RP_3M) {
+      double as[2], bs[4];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+      for (int rt = 0; rt < 2; ++rt) as[rt] = a[rt].re + a[rt].im;
 #pragma unroll
-      for (int ct = 0; ct < 4; ++ct) {
-        are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].re, b[ct].re, are[rt][ct], 0, 0, 0);
-        are[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[rt].im, b[ct].im, are[rt][ct], 0, 0, 0);
-        aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].re, b[ct].im, aim[rt][ct], 0, 0, 0);
-        aim[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].im, b[ct].re, aim[rt][ct], 0, 0, 0);
-      }
+      for (int ct = 0; ct < 4; ++ct) bs[ct] = b[ct].re + b[ct].im;
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+          acc[0][rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].re, b[ct].re, acc[0][rt][ct], 0, 0, 0);
+          acc[1][rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].im, b[ct].im, acc[1][rt][ct], 0, 0, 0);
+          acc[NP - 1][rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[rt], bs[ct], acc[NP - 1][rt][ct], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+          acc[0][rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].re, b[ct].re, acc[0][rt][ct], 0, 0, 0);
+          acc[0][rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[rt].im, b[ct].im, acc[0][rt][ct], 0, 0, 0);
+          acc[1][rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].re, b[ct].im, acc[1][rt][ct], 0, 0, 0);
+          acc[1][rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt].im, b[ct].re, acc[1][rt][ct], 0, 0, 0);
+        }
+    }
   }
 #pragma unroll
   for (int rt = 0; rt < 2; ++rt)
@@ -91,7 +114,9 @@ __global__ void __launch_bounds__(kRmThreads) k_reducedpot_mfma(int M, int K, co
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const long long x = x0 + 16 * rt + 4 * v + lk;
-        if (x < M && c < K) C[x + (long long)M * c] = mk(are[rt][ct][v], aim[rt][ct][v]);
+        const double p1 = acc[0][rt][ct][v], p2 = acc[1][rt][ct][v];
+        const z64 r = SDV_RP_3M ? mk(p1 - p2, acc[NP - 1][rt][ct][v] - p1 - p2) : mk(p1, p2);
+        if (x < M && c < K) C[x + (long long)M * c] = r;
       }
     }
 }
