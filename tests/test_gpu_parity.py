@@ -126,6 +126,25 @@ def test_op_tinv_all_circulant_sizes(orc, torch_dev, n):
     assert e32 < 2e-5
 
 
+@pytest.mark.parametrize("n", [70, 77])
+def test_op_tinv_partial_grid_large_circulant(orc, torch_dev, n):
+    """Grid edges below the circulant half-edge at the two largest sizes
+    (70 -> m = 72, 77 -> m = 81): the FP64 y/z kernel keeps the even y
+    channel's output of the rows that fit in the LDS beyond the plane and
+    parks the other rows in the plane's own HBM slots."""
+    from sincdvr import SincDVR
+    dx = 0.3
+    rng = np.random.default_rng(n)
+    den = np.asfortranarray(rng.standard_normal((n ** 3, 2)) + 1j * rng.standard_normal((n ** 3, 2)))
+    say("op_tinv n=%d (m > n)" % n)
+    s = SincDVR(n, dx, 2)
+    assert s.circulant_edge == {70: 144, 77: 162}[n]
+    ref = orc.op_tinv(n, orc.coulomb_kernel(n, dx), den)
+    assert relerr(s.op_tinv(den), ref) < 1e-12
+    s.set_precision(1)
+    assert relerr(s.op_tinv(den), ref) < 2e-5
+
+
 @pytest.mark.parametrize("n", [41, 21, 7, 81])
 @pytest.mark.parametrize("ecs", [False, True])
 def test_mult_ke(orc, torch_dev, ecs, n):
