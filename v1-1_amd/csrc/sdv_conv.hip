@@ -193,6 +193,24 @@ constexpr size_t yz_lds_bytes(int R, int es) {
 }
 
 template <typename T, int P, int Q>
+struct YZCfg;
+// Wavefronts per SIMD the y/z kernel is compiled for: one workgroup's
+// (NT / 256), except FP32 at m = 81, compiled for 6 -- two workgroups of 12
+// waves per CU (the plane, its tables and the spectrum slice take 81 KB of
+// LDS, so two fit the 160 KB).  That holds it to 80 VGPRs with 52 bytes of
+// spill, and measured at 81^3 the second workgroup wins: 1.40-1.43 ->
+// 1.33-1.35 ms per 128-convolution launch.  Smaller planes already fit
+// several workgroups by LDS, and the same cap only adds spills there (the
+// 64-set batch 216k -> 192k pair potentials/s, 41^3 275k -> 230k).
+#ifndef SDV_YZ_WPE32
+#define SDV_YZ_WPE32 6
+#endif
+template <typename T, int P, int Q>
+constexpr int yz_wpe() {
+  return sizeof(T) == 4 && P * Q == 81 ? SDV_YZ_WPE32 : (YZCfg<T, P, Q>::NT + 255) / 256;
+}
+
+template <typename T, int P, int Q>
 struct YZCfg {
   static constexpr int R = P * Q;
   static constexpr int G = P > Q ? P : Q;
@@ -735,7 +753,7 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
 // range, the plane body in a function of its own) measured 2.83 -> 3.20 ms in
 // FP64 at 81^3, the hardware's dynamic workgroup dispatch balancing better.
 template <typename T, int P, int Q, bool FULL>
-__global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz(KArgs<T> a) {
+__global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) __attribute__((amdgpu_waves_per_eu(yz_wpe<T, P, Q>()))) k_yz(KArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const unsigned L = xcd_span(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
   const unsigned plane = L / gridDim.y, pair = L % gridDim.y;
@@ -1064,7 +1082,7 @@ __global__ void __launch_bounds__(XThreads<T>::batch) k_xfwd_grp(const BSetDev<T
 }
 
 template <typename T, int P, int Q, bool FULL>
-__global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) k_yz_grp(const BSetDev<T>* __restrict__ sets,
+__global__ void __launch_bounds__((YZCfg<T, P, Q>::NT)) __attribute__((amdgpu_waves_per_eu(yz_wpe<T, P, Q>()))) k_yz_grp(const BSetDev<T>* __restrict__ sets,
                                                                const BItemDev* __restrict__ items,
                                                                const BWork* __restrict__ work, cplx<T>* X) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
