@@ -175,7 +175,9 @@ def summarize(out_dir, prof_dir, tag="pairpot_81_n16"):
 SQ_COUNTERS = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
                "SQ_LDS_IDX_ACTIVE", "SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES")
 SIMDS = 1024                  # 256 CUs x 4 SIMDs
-YZ_WAVES_PER_SIMD = 3         # 9x9 y/z workgroup: 768 threads, one workgroup per CU (LDS)
+# resident waves per SIMD of the 9x9 y/z kernel (768-thread workgroups): FP64 one
+# workgroup per CU (LDS), FP32 two (compiled for 6 waves per SIMD)
+YZ_WAVES_PER_SIMD = {0: 3, 1: 6}
 
 
 def summarize_sq(out_dir, prof_dir):
@@ -195,9 +197,11 @@ def summarize_sq(out_dir, prof_dir):
         "    python3 v1-1_amd/sincdvr/profiling.py sq OUT profiles", "",
         "Per dispatch of the full %d-convolution grid (the largest grid of each kernel), summed over the" % CONV_PER_LAUNCH,
         "device.  Waits are fractions of wavefront cycles; bank conflicts are per LDS-active cycle.",
-        "VALU busy = VALU instructions (one quad cycle each) over the SIMDs' quad cycles, with the",
-        "dispatch's wavefront cycles spread over %d SIMDs at the resident waves per SIMD (%d for k_yz)."
-        % (SIMDS, YZ_WAVES_PER_SIMD), "",
+        "VALU rate = VALU instructions per SIMD quad cycle, with the",
+        "dispatch's wavefront cycles spread over %d SIMDs at the resident waves per SIMD (for k_yz %d in"
+        % (SIMDS, YZ_WAVES_PER_SIMD[0]), "FP64, %d in FP32).  An FP64 instruction takes a quad cycle, so in FP64 the rate is the" % YZ_WAVES_PER_SIMD[1],
+        "VALU busy fraction; unpacked FP32 issues at up to twice that rate on gfx950, so the FP32",
+        "rate is no busy fraction (rocprofv3's VALUBusy, profiles/utilisation_81_n16.md, is).", "",
         "| kernel | VALU insts | LDS insts | wait any / wave cycles | wait issue / wave cycles | "
         "LDS bank conflict / LDS active |",
         "|---|---:|---:|---:|---:|---:|"]
@@ -218,16 +222,16 @@ def summarize_sq(out_dir, prof_dir):
                 c["SQ_WAIT_INST_ANY"].get(k, 0.0) / wc if wc else 0.0,
                 c["SQ_LDS_BANK_CONFLICT"].get(k, 0.0) / lds_act if lds_act else 0.0))
             if k.startswith("sdv::k_yz<") and wc:
-                busy = (c["SQ_INSTS_VALU"][k] / SIMDS) / (wc / (SIMDS * YZ_WAVES_PER_SIMD))
+                busy = (c["SQ_INSTS_VALU"][k] / SIMDS) / (wc / (SIMDS * YZ_WAVES_PER_SIMD[p]))
                 if p == 0:
                     lds_act = c["SQ_LDS_IDX_ACTIVE"].get(k, 0.0)
-                    yz64 = {"valu_busy": busy, "waves_per_cu": 4 * YZ_WAVES_PER_SIMD,
+                    yz64 = {"valu_busy": busy, "waves_per_cu": 4 * YZ_WAVES_PER_SIMD[0],
                             "lds_bank_conflict_frac": (c["SQ_LDS_BANK_CONFLICT"].get(k, 0.0) / lds_act
                                                        if lds_act else None),
                             "wait_any_frac": c["SQ_WAIT_ANY"].get(k, 0.0) / wc,
                             "source": "rocprofv3 SQ counters (profiles/k_yz_81_n16_sq_counters.md)"}
         if busy is not None:
-            lines.append("|  (VALU busy %s) | %.2f | | | | |" % (label, busy))
+            lines.append("|  (k_yz VALU instructions per SIMD quad cycle, %s) | %.2f | | | | |" % (label, busy))
     os.makedirs(prof_dir, exist_ok=True)
     with open(os.path.join(prof_dir, "k_yz_81_n16_sq_counters.md"), "w") as f:
         f.write("\n".join(lines) + "\n")
