@@ -472,7 +472,11 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
   // read left between the plane stores waits for every store before it;
   // 1.54 -> 1.51 ms at 81^3).  FP64 reads them in the store loop: its
   // registers hold the column input and accumulator, and hoisting spills.
-  constexpr bool HOIST = sizeof(T) == 4;
+  // FP32 compiled for two workgroups per CU (m = 81, 80 VGPRs) reads them in
+  // the store loop too, as FP64 does (spill 52 -> 20 bytes with the column
+  // twiddles below also left in LDS; 1.34-1.38 -> 1.29-1.32 ms).
+  constexpr bool TWO_WG = sizeof(T) == 4 && yz_wpe<T, P, Q>() > 3;
+  constexpr bool HOIST = sizeof(T) == 4 && !TWO_WG;
   auto t2_load = [&](cplx<T>* t2v, bool inv) {
 #pragma unroll
     for (int k = 0; k < P; ++k) t2v[k] = inv ? T2[k * P + li] : T2[li * P + k];
@@ -574,8 +578,8 @@ __device__ __forceinline__ void body_yz(const V& a, cplx<T>* base, int plane, un
     }
     // FP32 with P = Q: the nine T2 twiddles of this lane serve the forward and
     // the inverse of both channels (T2[a P + b] is symmetric in a, b), so they
-    // are read from LDS once per column
-    constexpr bool T2C = sizeof(T) == 4 && P == Q;
+    // are read from LDS once per column (not with TWO_WG)
+    constexpr bool T2C = sizeof(T) == 4 && P == Q && !TWO_WG;
     constexpr bool H2 = HOIST && !T2C;
     cplx<T> t2c[T2C ? P : 1];
     if (T2C && li < Q) {
