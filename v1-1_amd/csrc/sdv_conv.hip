@@ -20,7 +20,9 @@
 //
 // Each m-point line FFT is a two-step Cooley-Tukey (P-point DFT in registers,
 // twiddle, LDS exchange, Q-point DFT in registers); frequency k1 + P k2 is
-// kept at position k1 Q + k2 and the inverse consumes that order directly.
+// kept at position k1 Q + k2 (the x planes, and the y positions of the FP32
+// y/z plane) or at k1 + P k2 (the y positions of the FP64 y/z plane), and
+// the inverse consumes that order directly.
 #include <algorithm>
 #include <cstring>
 #include <type_traits>
