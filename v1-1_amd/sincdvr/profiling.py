@@ -11,7 +11,7 @@ summarize: reads rocprofv3 output (kernel-trace stats CSV, counter CSVs of a
      FETCH_SIZE pass and a WRITE_SIZE pass) and writes a short markdown summary
      and the traffic figure the benchmark reports.
 
-    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py --no-cpu --no-extra --steps 5 --warmup 2
+    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py --no-cpu --no-extra --steps 20 --warmup 20
     rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- python3 v1-1_amd/sincdvr/profiling.py run
     rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- python3 v1-1_amd/sincdvr/profiling.py run
     python3 v1-1_amd/sincdvr/profiling.py summarize OUT profiles
@@ -159,7 +159,7 @@ def summarize(out_dir, prof_dir, tag="pairpot_81_n16"):
     with open(os.path.join(prof_dir, f"{tag}_profile_summary.md"), "w") as f:
         f.write(f"# rocprofv3 summary: {tag}\n\n")
         f.write("Commands (repository root, on the GPU):\n\n")
-        f.write("    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py --no-cpu --no-extra --steps 5 --warmup 2\n")
+        f.write("    rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- python3 bench.py --no-cpu --no-extra --steps 20 --warmup 20\n")
         f.write("    rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- python3 v1-1_amd/sincdvr/profiling.py run\n")
         f.write("    rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- python3 v1-1_amd/sincdvr/profiling.py run\n")
         f.write("    python3 v1-1_amd/sincdvr/profiling.py summarize OUT profiles\n\n")
